@@ -1,0 +1,264 @@
+"""C-ABI table of the level-set time-stepping path and a ctypes binder for it.
+
+Every entry point keeps the Fortran calling convention of the reference
+(``src/*/lsm_*3d*.h``): all arguments are pointers, arrays are column-major and
+addressed from the first element of their own ghost box, boxes are six ints
+(ilo, ihi, jlo, jhi, klo, khi).  ``SPECS`` lists the arguments of each symbol
+in order with a kind tag:
+
+  R   real array                  (device pointer in liblsmhip)
+  B   box: six ints               (host)
+  S   real scalar, input          (host)
+  SO  real scalar, output         (host)
+  I   int scalar, input           (host)
+  IO  int scalar, output          (host)
+  IH  int array on the host       (n_lo / n_hi level tables)
+  IA  int array                   (device pointer in liblsmhip)
+  M   uint8 array                 (device pointer in liblsmhip)
+  U8  uint8 scalar, input         (host)
+
+``bind(lib, real)`` returns an object whose attributes are callables taking the
+arguments in spec order: arrays as raw addresses, numpy arrays or torch
+tensors, boxes as 6-tuples, scalars as Python numbers; scalar outputs (SO, IO)
+are passed as ``None`` and returned (a single value, or a tuple in order).
+"""
+from __future__ import annotations
+
+import ctypes
+
+_RAW = {
+    # --- spatial derivatives, src/toolbox/lsm_spatial_derivatives3d.h ---------
+    "lsm3dhjeno1_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 B:fb S:dx S:dy S:dz",
+    "lsm3dhjeno2_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 R:D2 B:D2 B:fb S:dx S:dy S:dz",
+    "lsm3dhjeno3_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 R:D2 B:D2 R:D3 B:D3 B:fb "
+                    "S:dx S:dy S:dz",
+    "lsm3dhjweno5_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 B:fb S:dx S:dy S:dz",
+    "lsm3dupwindhjeno1_": "R:px R:py R:pz B:g R:phi B:phi R:vx R:vy R:vz B:vel R:D1 B:D1 B:fb S:dx S:dy S:dz",
+    "lsm3dupwindhjeno2_": "R:px R:py R:pz B:g R:phi B:phi R:vx R:vy R:vz B:vel R:D1 B:D1 R:D2 B:D2 B:fb S:dx S:dy S:dz",
+    "lsm3dupwindhjeno3_": "R:px R:py R:pz B:g R:phi B:phi R:vx R:vy R:vz B:vel R:D1 B:D1 R:D2 B:D2 R:D3 B:D3 B:fb "
+                          "S:dx S:dy S:dz",
+    "lsm3dupwindhjweno5_": "R:px R:py R:pz B:g R:phi B:phi R:vx R:vy R:vz B:vel R:D1 B:D1 B:fb S:dx S:dy S:dz",
+    "lsm3dcentralgradorder2_": "R:px R:py R:pz B:g R:phi B:phi B:fb S:dx S:dy S:dz",
+    "lsm3dcentralgradorder4_": "R:px R:py R:pz B:g R:phi B:phi B:fb S:dx S:dy S:dz",
+    "lsm3dlaplacianorder2_": "R:lap B:lap R:phi B:phi B:fb S:dx S:dy S:dz",
+    "lsm3daveragegradphi_": "R:px R:py R:pz B:g R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm B:fb",
+    "lsm3dgradientmagnitude_": "R:px R:py R:pz R:mag B:g B:fb",
+    # --- RHS terms, src/toolbox/lsm_level_set_evolution3d.h --------------------
+    "lsm3dzerooutlevelseteqnrhs_": "R:rhs B:rhs",
+    "lsm3daddadvectiontermtolserhs_": "R:rhs B:rhs R:px R:py R:pz B:g R:vx R:vy R:vz B:vel B:fb",
+    "lsm3daddnormalveltermtolserhs_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:vn B:vel B:fb",
+    "lsm3daddconstnormalveltermtolserhs_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm S:vn B:fb",
+    "lsm3daddconstcurvtermtolserhs_": "R:rhs B:rhs R:px R:py R:pz B:g R:pxx R:pxy R:pxz R:pyy R:pyz R:pzz B:g2 S:b B:fb",
+    "lsm3daddconstprecomputedcurvtermtolserhs_": "R:rhs B:rhs R:kappa B:kappa R:gmag B:gmag S:b B:fb",
+    "lsm3daddexternalandnormalveltermtolserhs_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:vn "
+                                                 "R:vx R:vy R:vz B:vel B:fb",
+    # --- reinitialization / curvature ------------------------------------------
+    "lsm3dcomputereinitializationeqnrhs_": "R:rhs B:rhs R:phi B:phi R:phi0 B:phi0 R:pxp R:pyp R:pzp B:gp R:pxm R:pym "
+                                           "R:pzm B:gm B:fb S:dx S:dy S:dz I:use_phi0",
+    "lsm3dcomputemeancurvatureorder2_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g B:fb S:dx S:dy S:dz",
+    # --- TVD Runge-Kutta, src/toolbox/lsm_tvd_runge_kutta3d.h -----------------
+    "lsm3drk1step_": "R:un B:un R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    "lsm3dtvdrk2stage1_": "R:un B:un R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    "lsm3dtvdrk2stage2_": "R:un B:un R:u1 B:u1 R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    "lsm3dtvdrk3stage1_": "R:un B:un R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    "lsm3dtvdrk3stage2_": "R:u2 B:u2 R:u1 B:u1 R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    "lsm3dtvdrk3stage3_": "R:un B:un R:u2 B:u2 R:uc B:uc R:rhs B:rhs B:fb S:dt",
+    # --- reductions ---------------------------------------------------------------
+    "lsm3dmaxnormdiff_": "SO:out R:f1 B:f1 R:f2 B:f2 B:ib",
+    "lsm3dcomputestableadvectiondt_": "SO:dt R:vx R:vy R:vz B:vel B:ib S:dx S:dy S:dz S:cfl",
+    "lsm3dcomputestablenormalveldt_": "SO:dt R:vn B:vel R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm B:ib "
+                                      "S:dx S:dy S:dz S:cfl",
+    "lsm3dcomputestableconstnormalveldt_": "SO:dt S:vn R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm B:ib "
+                                           "S:dx S:dy S:dz S:cfl",
+    "lsm3dvolumeintegralphilessthanzero_": "SO:int R:F B:F R:phi B:phi B:ib S:dx S:dy S:dz S:eps",
+    "lsm3dvolumeintegralphigreaterthanzero_": "SO:int R:F B:F R:phi B:phi B:ib S:dx S:dy S:dz S:eps",
+    "lsm3dsurfaceintegral_": "SO:int R:F B:F R:phi B:phi R:px R:py R:pz B:g B:ib S:dx S:dy S:dz S:eps",
+    "lsm3dvolumeregionphilessthanzero_": "SO:vol R:phi B:phi B:ib S:dx S:dy S:dz S:eps",
+    "lsm3dvolumeregionphigreaterthanzero_": "SO:vol R:phi B:phi B:ib S:dx S:dy S:dz S:eps",
+    "lsm3dsurfaceareazerolevelset_": "SO:area R:phi B:phi R:px R:py R:pz B:g B:ib S:dx S:dy S:dz S:eps",
+    # --- ghost fills, src/boundary_conditions/lsm_boundary_conditions3d.h -------
+    "lsm3dlinearextrapolation_": "R:phi B:gb B:fb I:idx",
+    "lsm3dsignedlinearextrapolation_": "R:phi B:gb B:fb I:idx",
+    "lsm3dcopyextrapolation_": "R:phi B:gb B:fb I:idx",
+    "lsm3dhomogeneousneumanneno1_": "R:phi B:gb B:fb I:idx",
+    "lsm3dhomogeneousneumanneno2_": "R:phi B:gb B:fb I:idx",
+    "lsm3dhomogeneousneumanneno3_": "R:phi B:gb B:fb I:idx",
+    "lsm3dhomogeneousneumannweno5_": "R:phi B:gb B:fb I:idx",
+    # --- smoothed Heaviside / delta, src/toolbox/lsm_calculus_toolbox.f --------
+    "lsmheaviside_": "SO:H S:x S:eps",
+    "lsmdeltafunction_": "SO:delta S:x S:eps",
+    # --- narrow-band derivatives, src/toolbox/lsm_spatial_derivatives3d_local.h -
+    "lsm3dhjeno1local_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 S:dx S:dy S:dz "
+                         "IA:ix IA:iy IA:iz I:nlo0 I:nhi0 I:nlo1 I:nhi1 M:nb B:nb U8:mark_fb U8:mark_D1",
+    "lsm3dhjeno2local_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 R:D2 B:D2 "
+                         "S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 M:nb B:nb "
+                         "U8:mark_fb U8:mark_D1 U8:mark_D2",
+    "lsm3dhjweno5local_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 S:dx S:dy S:dz "
+                          "IA:ix IA:iy IA:iz I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 I:nlo3 I:nhi3 M:nb B:nb "
+                          "U8:mark_fb U8:mark_D1",
+    "lsm3dcentralgradorder2local_": "R:px R:py R:pz B:g R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi "
+                                    "M:nb B:nb U8:mark_fb",
+    "lsm3dcentralgradorder4local_": "R:px R:py R:pz B:g R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi "
+                                    "M:nb B:nb U8:mark_fb",
+    "lsm3dlaplacianorder2local_": "R:lap B:lap R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb "
+                                  "U8:mark_fb",
+    "lsm3dcomputeavegradphilocal_": "SO:ave R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb "
+                                    "U8:mark_fb",
+    "lsm3dgradientmagnitudelocal_": "R:px R:py R:pz R:mag B:g IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    # --- narrow-band RHS, src/toolbox/lsm_level_set_evolution3d_local.h --------
+    "lsm3dzerooutlevelseteqnrhslocal_": "R:rhs B:rhs IA:ix IA:iy IA:iz I:nlo I:nhi",
+    "lsm3daddadvectiontermtolserhslocal_": "R:rhs B:rhs R:px R:py R:pz B:g R:vx R:vy R:vz B:vel IA:ix IA:iy IA:iz "
+                                           "I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3daddnormalveltermtolserhslocal_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:vn B:vel "
+                                           "IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3daddconstnormalveltermtolserhslocal_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm S:vn "
+                                                "IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3daddconstcurvtermtolserhslocal_": "R:rhs B:rhs R:px R:py R:pz B:g R:pxx R:pxy R:pxz R:pyy R:pyz R:pzz B:g2 "
+                                           "S:b IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3daddconstprecomputedcurvtermtolserhslocal_": "R:rhs B:rhs R:kappa B:kappa R:gmag B:gmag S:b IA:ix IA:iy "
+                                                      "IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3daddexternalandnormalveltermtolserhslocal_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm "
+                                                      "R:vn R:vx R:vy R:vz B:vel IA:ix IA:iy IA:iz I:nlo I:nhi M:nb "
+                                                      "B:nb U8:mark_fb",
+    # --- narrow-band Runge-Kutta, src/toolbox/lsm_tvd_runge_kutta3d_local.h ----
+    "lsm3drk1steplocal_": "R:un B:un R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dtvdrk2stage1local_": "R:un B:un R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb "
+                               "U8:mark_fb",
+    "lsm3dtvdrk2stage2local_": "R:un B:un R:u1 B:u1 R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb "
+                               "B:nb U8:mark_fb",
+    "lsm3dtvdrk3stage1local_": "R:un B:un R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb "
+                               "U8:mark_fb",
+    "lsm3dtvdrk3stage2local_": "R:u2 B:u2 R:u1 B:u1 R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb "
+                               "B:nb U8:mark_fb",
+    "lsm3dtvdrk3stage3local_": "R:un B:un R:u2 B:u2 R:uc B:uc R:rhs B:rhs S:dt IA:ix IA:iy IA:iz I:nlo I:nhi M:nb "
+                               "B:nb U8:mark_fb",
+    # --- narrow-band reductions, src/toolbox/lsm_math_utils3d_local.h ----------
+    "lsm3dmaxnormdifflocal_": "SO:out R:f1 B:f1 R:f2 B:f2 IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputestableadvectiondtlocal_": "SO:dt R:vx R:vy R:vz B:vel S:dx S:dy S:dz S:cfl IA:ix IA:iy IA:iz I:nlo "
+                                           "I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputestablenormalveldtlocal_": "SO:dt R:vn B:vel R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm S:dx "
+                                           "S:dy S:dz S:cfl IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputestableconstnormalveldtlocal_": "SO:dt S:vn R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm S:dx "
+                                                "S:dy S:dz S:cfl IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputereinitializationeqnrhslocal_": "R:rhs B:rhs R:phi B:phi R:phi0 B:phi0 R:pxp R:pyp R:pzp B:gp "
+                                                "R:pxm R:pym R:pzm B:gm S:dx S:dy S:dz I:use_phi0 IA:ix IA:iy "
+                                                "IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputemeancurvatureorder2local_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g S:dx S:dy S:dz "
+                                             "IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    # --- band construction, src/toolbox/lsm_localization3d.h --------------------
+    "lsm3ddeterminenarrowband_": "R:phi B:phi M:nb B:nb IA:ix IA:iy IA:iz I:nlo I:nhi IH:n_lo IH:n_hi IA:outer "
+                                 "I:nlo_outer I:nhi_outer IO:nlo_outer_plus IO:nhi_outer_plus IO:nlo_outer_minus "
+                                 "IO:nhi_outer_minus S:width S:width_inner I:level",
+    "lsm3dmarknarrowbandboundarylayer_": "M:nb B:gb B:fb U8:mark",
+    "lsm3ddeterminenarrowbandfrommask_": "R:mask B:mask M:nb B:nb IA:ix IA:iy IA:iz I:nlo I:nhi IH:n_lo IH:n_hi "
+                                         "I:level I:use_mask_sign",
+    "lsm3dmultiplycutofflserhslocal_": "R:phi R:rhs B:gb IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb S:beta "
+                                       "S:gamma",
+    "lsm3dcheckouternarrowbandlayer_": "IO:change_sign R:phi B:gb IA:ix IA:iy IA:iz I:nlo I:nhi IA:outer "
+                                       "I:nlo_outer I:nhi_outer I:nlo_outer_plus I:nhi_outer_plus "
+                                       "I:nlo_outer_minus I:nhi_outer_minus",
+    "lsm3dimposemasklocal_": "R:dest B:dest R:src B:src R:mask B:mask IA:ix IA:iy IA:iz I:nlo I:nhi",
+    "lsm3dcopydatalocal_": "R:dest B:dest R:src B:src IA:ix IA:iy IA:iz I:nlo I:nhi",
+}
+
+# Entry points with no reference counterpart (periodic fill is absent from the
+# serial reference; the 3D WENO5 local routine is authored from the 2D one).
+AUTHORED = {"lsmhip_periodic_face_", "lsm3dhjweno5local_"}
+_RAW["lsmhip_periodic_face_"] = "R:phi B:gb B:fb I:idx"
+
+
+def _parse(spec: str):
+    out = []
+    for tok in spec.split():
+        kind, name = tok.split(":")
+        out.append((kind, name))
+    return out
+
+
+SPECS = {name: _parse(s) for name, s in _RAW.items()}
+
+
+def _addr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):  # torch tensor
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):  # numpy array
+        return x.ctypes.data
+    raise TypeError("cannot take the address of %r" % type(x))
+
+
+class _Fn:
+    def __init__(self, cfn, spec, real):
+        self._f = cfn
+        self._spec = spec
+        self._real = real
+
+    def __call__(self, *args):
+        if len(args) != len(self._spec):
+            raise TypeError("expected %d arguments, got %d" % (len(self._spec), len(args)))
+        keep, cargs, outs = [], [], []
+        for (kind, name), a in zip(self._spec, args):
+            if kind == "B":
+                if len(a) != 6:
+                    raise ValueError("box %s needs 6 ints" % name)
+                for v in a:
+                    c = ctypes.c_int(int(v))
+                    keep.append(c)
+                    cargs.append(ctypes.byref(c))
+            elif kind == "S":
+                c = self._real(float(a))
+                keep.append(c)
+                cargs.append(ctypes.byref(c))
+            elif kind == "SO":
+                c = self._real(0.0)
+                keep.append(c)
+                outs.append(c)
+                cargs.append(ctypes.byref(c))
+            elif kind == "I":
+                c = ctypes.c_int(int(a))
+                keep.append(c)
+                cargs.append(ctypes.byref(c))
+            elif kind == "IO":
+                c = ctypes.c_int(0 if a is None else int(a))
+                keep.append(c)
+                outs.append(c)
+                cargs.append(ctypes.byref(c))
+            elif kind == "U8":
+                c = ctypes.c_ubyte(int(a))
+                keep.append(c)
+                cargs.append(ctypes.byref(c))
+            else:  # R, IA, IH, M: addresses
+                cargs.append(ctypes.c_void_p(_addr(a)))
+        self._f(*cargs)
+        vals = tuple(o.value for o in outs)
+        if not vals:
+            return None
+        return vals[0] if len(vals) == 1 else vals
+
+
+class Bound:
+    """Callables for every SPECS symbol a library exports."""
+
+    def __init__(self, lib, real=ctypes.c_double, prefix="", strip_underscore=False, names=None):
+        self.lib = lib
+        self.missing = []
+        for sym, spec in SPECS.items():
+            if names is not None and sym not in names:
+                continue
+            cname = prefix + (sym[:-1] if strip_underscore else sym)
+            try:
+                cfn = getattr(lib, cname)
+            except AttributeError:
+                self.missing.append(sym)
+                continue
+            nargs = sum(6 if k == "B" else 1 for k, _ in spec)
+            cfn.argtypes = None
+            cfn.restype = None
+            setattr(self, sym.rstrip("_"), _Fn(cfn, spec, real))
+
+
+def arg_count(sym):
+    return sum(6 if k == "B" else 1 for k, _ in SPECS[sym])

@@ -1,0 +1,78 @@
+/*
+ * lsm_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of LSMLIB's 3D level-set time-stepping kernels, used by
+ * tests/, bench.py's cpu_baseline leg and __graft_entry__.smoke() as the
+ * checker of the HIP library.  Nothing in the product (lsmlib_amd/) links,
+ * loads or calls this code.
+ *
+ * Every entry point keeps the reference's Fortran ABI (all arguments by
+ * pointer, column-major arrays addressed from their own ghost box, fill box
+ * and scalars separate) and is exported as `oracle_<fortran symbol>`, e.g.
+ * oracle_lsm3dhjweno5 for lsm3dhjweno5_.  Arithmetic is FP64, compiled with
+ * -ffp-contract=off, and follows the reference's operation order so that the
+ * results are bitwise comparable with the reference kernels.
+ */
+#ifndef LSM_ORACLE_H
+#define LSM_ORACLE_H
+
+#include <math.h>
+#include <stdint.h>
+
+/* Template constants of the double-precision build (src/CMakeLists.txt:10-20). */
+#define ORC_ZERO_TOL 1.e-11
+#define ORC_TINY 1.e-36
+#define ORC_BIG 1.e10
+
+/* Six inclusive box bounds passed by pointer, as in every reference signature. */
+#define BOXP(n)                                                          \
+  const int *ilo_##n, const int *ihi_##n, const int *jlo_##n,            \
+      const int *jhi_##n, const int *klo_##n, const int *khi_##n
+
+typedef struct {
+  double *p;
+  long ilo, jlo, klo, nx, nxy;
+} orc_arr;
+
+typedef struct {
+  unsigned char *p;
+  long ilo, jlo, klo, nx, nxy;
+} orc_mask;
+
+static inline orc_arr orc_view(const double *p, const int *ilo, const int *ihi,
+                               const int *jlo, const int *jhi, const int *klo) {
+  orc_arr a;
+  a.p = (double *)p;
+  a.ilo = *ilo;
+  a.jlo = *jlo;
+  a.klo = *klo;
+  a.nx = (long)(*ihi - *ilo + 1);
+  a.nxy = a.nx * (long)(*jhi - *jlo + 1);
+  return a;
+}
+
+static inline orc_mask orc_mview(const unsigned char *p, const int *ilo, const int *ihi,
+                                 const int *jlo, const int *jhi, const int *klo) {
+  orc_mask a;
+  a.p = (unsigned char *)p;
+  a.ilo = *ilo;
+  a.jlo = *jlo;
+  a.klo = *klo;
+  a.nx = (long)(*ihi - *ilo + 1);
+  a.nxy = a.nx * (long)(*jhi - *jlo + 1);
+  return a;
+}
+
+#define VIEW(p, n) orc_view((p), ilo_##n, ihi_##n, jlo_##n, jhi_##n, klo_##n)
+#define MVIEW(p, n) orc_mview((p), ilo_##n, ihi_##n, jlo_##n, jhi_##n, klo_##n)
+#define AT(a, i, j, k) \
+  ((a).p[((long)(i) - (a).ilo) + (a).nx * ((long)(j) - (a).jlo) + (a).nxy * ((long)(k) - (a).klo)])
+
+/* Fortran integer*1 marks are compared as signed bytes. */
+#define MARK(a, i, j, k) ((signed char)AT(a, i, j, k))
+
+static inline double orc_sq(double x) { return x * x; }
+static inline double orc_max(double a, double b) { return a > b ? a : b; }
+static inline double orc_min(double a, double b) { return a < b ? a : b; }
+
+#endif

@@ -1,0 +1,88 @@
+"""Pin the oracle (plain-C restatement) bitwise against the reference's own
+FP64 Fortran kernels, compiled by oracle/Makefile into oracle/_ref/.
+
+Runs on the CPU.  Skipped when the reference build is absent.
+"""
+import numpy as np
+import pytest
+
+import harness as H
+from lsmlib_amd import capi
+
+pytestmark = pytest.mark.skipif(H.reference() is None, reason="oracle/_ref/liblsm_ref.so not built")
+
+WHOLE_GRID = [
+    "lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_",
+    "lsm3dupwindhjeno1_", "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_", "lsm3dupwindhjweno5_",
+    "lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_", "lsm3daveragegradphi_",
+    "lsm3dgradientmagnitude_", "lsm3dzerooutlevelseteqnrhs_", "lsm3daddadvectiontermtolserhs_",
+    "lsm3daddnormalveltermtolserhs_", "lsm3daddconstnormalveltermtolserhs_", "lsm3daddconstcurvtermtolserhs_",
+    "lsm3dcomputereinitializationeqnrhs_", "lsm3dcomputemeancurvatureorder2_",
+    "lsm3drk1step_", "lsm3dtvdrk2stage1_", "lsm3dtvdrk2stage2_", "lsm3dtvdrk3stage1_", "lsm3dtvdrk3stage2_",
+    "lsm3dtvdrk3stage3_", "lsm3dmaxnormdiff_", "lsm3dcomputestableadvectiondt_",
+    "lsm3dcomputestablenormalveldt_", "lsm3dcomputestableconstnormalveldt_",
+    "lsm3dvolumeintegralphilessthanzero_", "lsm3dvolumeintegralphigreaterthanzero_", "lsm3dsurfaceintegral_",
+    "lsm3dvolumeregionphilessthanzero_", "lsm3dvolumeregionphigreaterthanzero_",
+    "lsm3dsurfaceareazerolevelset_",
+]
+QUIRKY_BOUNDS = ["lsm3daddconstprecomputedcurvtermtolserhs_", "lsm3daddexternalandnormalveltermtolserhs_"]
+BCS = ["lsm3dlinearextrapolation_", "lsm3dsignedlinearextrapolation_", "lsm3dcopyextrapolation_",
+       "lsm3dhomogeneousneumanneno1_", "lsm3dhomogeneousneumanneno2_", "lsm3dhomogeneousneumanneno3_",
+       "lsm3dhomogeneousneumannweno5_"]
+
+
+def test_oracle_exports_everything():
+    assert H.oracle().missing == []
+    ref_missing = set(H.reference().missing)
+    assert ref_missing == capi.AUTHORED, ref_missing
+
+
+def _prepare(case, sym):
+    # exact zeros in a gradient magnitude exercise the curvature's zero branch
+    if "gmag" in [n for _, n in capi.SPECS[sym]]:
+        g = case.array("gmag", H.box_of_array(sym)["gmag"])
+        g.reshape(-1)[::17] = 0.0
+    if sym.startswith("lsm3dcomputestableconstnormalveldt"):
+        case.scalars["vn"] = -0.75
+
+
+@pytest.mark.parametrize("sym", WHOLE_GRID)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_whole_grid_bitwise(sym, seed):
+    c = H.grid_case(seed)
+    _prepare(c, sym)
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("sym", ["lsm3daddconstnormalveltermtolserhs_", "lsm3dcomputereinitializationeqnrhs_"])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_sign_variants(sym, variant):
+    c = H.grid_case(7)
+    c.scalars["vn"] = -0.6 if variant else 1e-13
+    c.scalars["use_phi0"] = variant
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("sym", QUIRKY_BOUNDS)
+def test_quirky_loop_bounds(sym):
+    c = H.grid_case(3, mixed_boxes=False)
+    c.boxes["rhs"] = (-1, 19, -2, 14, -3, 10)
+    c.boxes["fb"] = (0, 17, 0, 13, 0, 9)
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("sym", BCS)
+@pytest.mark.parametrize("idx", range(6))
+def test_ghost_fill_faces(sym, idx):
+    c = H.grid_case(11, mixed_boxes=False)
+    c.boxes["gb"] = c.gb
+    c.scalars["idx"] = idx
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("x", [-0.5, -0.3, -0.29, -0.1, 0.0, 1e-9, 0.05, 0.2999, 0.3, 0.31])
+def test_heaviside_delta(x):
+    c = H.Case((0, 0, 0, 0, 0, 0), (0, 0, 0, 0, 0, 0))
+    c.scalars.update(x=x, eps=0.3)
+    for sym in ("lsmheaviside_", "lsmdeltafunction_"):
+        H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
