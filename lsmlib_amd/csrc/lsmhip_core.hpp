@@ -1,0 +1,136 @@
+// lsmhip_core.hpp -- shared types and runtime plumbing of the HIP level-set library.
+//
+// Arrays follow the reference's conventions: column-major, each array
+// addressed from the first element of its own ghost box (i fastest), boxes as
+// inclusive (ilo, ihi, jlo, jhi, klo, khi).  Array arguments of the exported
+// entry points are device pointers; scalars and boxes stay host pointers as in
+// the reference signatures.
+//
+// The library is compiled once per precision: LSMHIP_REAL=double gives
+// liblsmhip.so (the FP64 drop-in, operation order of the reference, no FMA
+// contraction), LSMHIP_REAL=float gives liblsmhip_f32.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#ifndef LSMHIP_REAL
+#define LSMHIP_REAL double
+#endif
+
+typedef LSMHIP_REAL real_t;
+
+namespace lsmhip {
+
+constexpr bool kDouble = sizeof(real_t) == 8;
+// Template constants of the reference build (src/CMakeLists.txt:10-20):
+// zero tolerance 1e-11 in double precision and 1e-5 in single precision,
+// tiny nonzero number 1e-36 in both.
+constexpr real_t kZeroTol = kDouble ? real_t(1.e-11) : real_t(1.e-5);
+constexpr real_t kTiny = real_t(1.e-36);
+constexpr real_t kBig = real_t(1.e10);
+
+struct Box {
+  int ilo, ihi, jlo, jhi, klo, khi;
+  __host__ __device__ int nx() const { return ihi - ilo + 1; }
+  __host__ __device__ int ny() const { return jhi - jlo + 1; }
+  __host__ __device__ int nz() const { return khi - klo + 1; }
+  __host__ __device__ bool empty() const { return ihi < ilo || jhi < jlo || khi < klo; }
+  __host__ __device__ bool has(int i, int j, int k) const {
+    return i >= ilo && i <= ihi && j >= jlo && j <= jhi && k >= klo && k <= khi;
+  }
+  bool contains(const Box& b) const {
+    return b.empty() || (b.ilo >= ilo && b.ihi <= ihi && b.jlo >= jlo && b.jhi <= jhi && b.klo >= klo && b.khi <= khi);
+  }
+  Box widen(int wx, int wy, int wz) const {
+    return Box{ilo - wx, ihi + wx, jlo - wy, jhi + wy, klo - wz, khi + wz};
+  }
+  __host__ __device__ long long volume() const { return empty() ? 0 : (long long)nx() * ny() * nz(); }
+};
+
+// Device view of one array: element (i,j,k) of an array whose ghost box
+// starts at (ilo,jlo,klo) with row stride sy and plane stride sz.
+template <class T>
+struct View {
+  T* p;
+  int ilo, jlo, klo, sy, sz;
+  __device__ __forceinline__ T& operator()(int i, int j, int k) const {
+    return p[(i - ilo) + sy * (j - jlo) + sz * (k - klo)];
+  }
+  __device__ __forceinline__ int offset(int i, int j, int k) const {
+    return (i - ilo) + sy * (j - jlo) + sz * (k - klo);
+  }
+};
+
+template <class T>
+inline View<T> view(T* p, const Box& b) {
+  return View<T>{p, b.ilo, b.jlo, b.klo, b.nx(), b.nx() * b.ny()};
+}
+
+inline Box box_of(const int* ilo, const int* ihi, const int* jlo, const int* jhi, const int* klo, const int* khi) {
+  return Box{*ilo, *ihi, *jlo, *jhi, *klo, *khi};
+}
+
+// ----- error side channel and stream state (runtime.hip) ---------------------
+enum ErrorCode {
+  kOk = 0,
+  kErrBox = 1,    // a fill/include box is not inside an array's ghost box
+  kErrHip = 2,    // a HIP runtime call failed
+  kErrArg = 3,    // invalid scalar argument (null pointer, bad index, ...)
+  kErrIndex = 4,  // an index-list entry (or its stencil) left an array's ghost box
+  kErrSize = 5,   // an array exceeds the 2^31-element addressing limit
+};
+
+void set_error(int code, const char* fmt, ...);
+hipStream_t stream();
+// Ends an entry point: records a launch error and, in synchronous mode (the
+// default, matching the reference's completion semantics), waits for the stream.
+void finish(const char* who);
+bool check_hip(hipError_t e, const char* who);
+// Device scratch owned by the library, grown on demand (never shrinks).
+void* workspace(size_t bytes);
+// Per-device int flag that index-list kernels raise when an index or a stencil
+// neighbour lies outside an array's ghost box (the point is then skipped).
+int* device_flag();
+// Reads and clears the device flag (synchronises the stream).
+int take_device_flag(const char* who);
+
+bool check_box(const Box& outer, const Box& inner, const char* who, const char* what);
+bool check_size(const Box& b, const char* who);
+
+// Launch shape for kernels that map one thread to one cell of a box.
+constexpr int kBX = 64, kBY = 4;
+inline dim3 grid_for(const Box& b) {
+  return dim3((unsigned)((b.nx() + kBX - 1) / kBX), (unsigned)((b.ny() + kBY - 1) / kBY), (unsigned)b.nz());
+}
+inline dim3 block2d() { return dim3(kBX, kBY, 1); }
+
+#define LSMHIP_CELL(b)                              \
+  const int i = (b).ilo + blockIdx.x * kBX + threadIdx.x; \
+  const int j = (b).jlo + blockIdx.y * kBY + threadIdx.y; \
+  const int k = (b).klo + blockIdx.z;               \
+  if (i > (b).ihi || j > (b).jhi) return
+
+// Exported-signature helpers: six box bounds passed by pointer.
+#define LSMHIP_BOXP(n) \
+  const int *ilo_##n, const int *ihi_##n, const int *jlo_##n, const int *jhi_##n, const int *klo_##n, const int *khi_##n
+#define LSMHIP_BOX(n) ::lsmhip::box_of(ilo_##n, ihi_##n, jlo_##n, jhi_##n, klo_##n, khi_##n)
+
+template <class T>
+__device__ __forceinline__ T sq(T x) {
+  return x * x;
+}
+template <class T>
+__device__ __forceinline__ T maxr(T a, T b) {
+  return a > b ? a : b;
+}
+template <class T>
+__device__ __forceinline__ T minr(T a, T b) {
+  return a < b ? a : b;
+}
+
+}  // namespace lsmhip
+
+#define LSMHIP_EXPORT extern "C" __attribute__((visibility("default")))

@@ -1,0 +1,276 @@
+// reductions3d.hip -- CFL time steps, max-norm difference, smoothed volume and
+// surface integrals (src/toolbox/lsm_math_utils3d.f.in,
+// src/geometry/lsm_geometry3d_fort.f.in).
+//
+// Maxima are order-independent, so the CFL steps and the max-norm difference
+// equal the reference's bitwise.  The reference accumulates integrals with a
+// naive serial sum; here each cell's term is computed as the reference
+// computes it and the terms are summed in FP64 with a fixed-order
+// compensated (Neumaier) reduction: deterministic, and at least as accurate.
+// Both precisions accumulate in FP64.
+#include "lsmhip_core.hpp"
+#include "reduce.hpp"
+#include "stencil.hpp"
+
+using namespace lsmhip;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+// Truncated 1/pi of the reference's Fortran Heaviside (lsm_geometry3d_fort.f.in:432-433).
+constexpr double kOneOverPi = 0.31830988618379;
+
+struct MaxNormDiff {
+  View<const real_t> a, b;
+  __device__ double operator()(int i, int j, int k) const { return (double)fabs(a(i, j, k) - b(i, j, k)); }
+};
+
+struct AdvectionCfl {
+  View<const real_t> u, v, w;
+  real_t ix, iy, iz;
+  __device__ double operator()(int i, int j, int k) const {
+    return (double)(fabs(u(i, j, k)) * ix + fabs(v(i, j, k)) * iy + fabs(w(i, j, k)) * iz);
+  }
+};
+
+struct NormalCfl {
+  View<const real_t> vn, xp, yp, zp, xm, ym, zm;
+  real_t vconst, mdx2, ix, iy, iz;
+  __device__ double operator()(int i, int j, int k) const {
+    const real_t av = vn.p ? fabs(vn(i, j, k)) : vconst;
+    return (double)stencil::normal_h_over_dx(av, xp(i, j, k), yp(i, j, k), zp(i, j, k), xm(i, j, k), ym(i, j, k),
+                                             zm(i, j, k), mdx2, ix, iy, iz);
+  }
+};
+
+// sign = -1: smoothed volume of {phi < 0}; sign = +1: of {phi > 0}.  F.p null means F = 1.
+struct VolumeTerm {
+  View<const real_t> phi, F;
+  real_t eps, dV;
+  int sign;
+  __device__ double operator()(int i, int j, int k) const {
+    const real_t p = phi(i, j, k), q = p / eps;
+    const real_t f = F.p ? F(i, j, k) : real_t(1);
+    if (sign < 0) {
+      if (p < -eps) return (double)(F.p ? f * dV : dV);
+      if (p < eps) {
+        const real_t h = real_t(0.5) * (real_t(1) - q - real_t(kOneOverPi) * sin(real_t(kPi) * q));
+        return (double)(F.p ? h * f * dV : h * dV);
+      }
+    } else {
+      if (p > eps) return (double)(F.p ? f * dV : dV);
+      if (p > -eps) {
+        const real_t h = real_t(0.5) * (real_t(1) + q + real_t(kOneOverPi) * sin(real_t(kPi) * q));
+        return (double)(F.p ? h * f * dV : h * dV);
+      }
+    }
+    return 0.0;
+  }
+};
+
+struct SurfaceTerm {
+  View<const real_t> phi, F, x, y, z;
+  real_t eps, ie, dV;
+  __device__ double operator()(int i, int j, int k) const {
+    const real_t p = phi(i, j, k);
+    if (!(fabs(p) < eps)) return 0.0;
+    const real_t delta = real_t(0.5) * ie * (real_t(1) + cos(real_t(kPi) * p * ie));
+    const real_t gx = x(i, j, k), gy = y(i, j, k), gz = z(i, j, k);
+    const real_t n = sqrt(gx * gx + gy * gy + gz * gz);
+    return (double)(F.p ? delta * n * F(i, j, k) * dV : delta * n * dV);
+  }
+};
+
+bool ok(const char* who, const Box& ib, std::initializer_list<std::pair<Box, const char*>> arrays) {
+  for (auto& a : arrays)
+    if (!check_box(a.first, ib, who, a.second) || !check_size(a.first, who)) return false;
+  return true;
+}
+
+real_t cfl_dt(double m, real_t cfl) { return cfl / (real_t(m) + kTiny); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+LSMHIP_EXPORT void lsm3dmaxnormdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP(g1), const real_t* f2,
+                                     LSMHIP_BOXP(g2), LSMHIP_BOXP(ib)) {
+  const char* who = "lsm3dmaxnormdiff_";
+  Box ib = LSMHIP_BOX(ib), g1 = LSMHIP_BOX(g1), g2 = LSMHIP_BOX(g2);
+  *out = real_t(0);
+  if (ib.empty() || !ok(who, ib, {{g1, "field1"}, {g2, "field2"}})) return;
+  double m = 0;
+  if (reduce::box_max(who, ib, MaxNormDiff{view(f1, g1), view(f2, g2)}, &m)) *out = real_t(m);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestableadvectiondt_(real_t* dt, const real_t* vx, const real_t* vy, const real_t* vz,
+                                                  LSMHIP_BOXP(vg), LSMHIP_BOXP(ib), const real_t* dx,
+                                                  const real_t* dy, const real_t* dz, const real_t* cfl) {
+  const char* who = "lsm3dcomputestableadvectiondt_";
+  Box ib = LSMHIP_BOX(ib), vg = LSMHIP_BOX(vg);
+  double m = -1.0;
+  if (!ib.empty()) {
+    if (!ok(who, ib, {{vg, "vel"}})) return;
+    AdvectionCfl op{view(vx, vg), view(vy, vg), view(vz, vg), real_t(1.0) / *dx, real_t(1.0) / *dy,
+                    real_t(1.0) / *dz};
+    if (!reduce::box_max(who, ib, op, &m)) return;
+  }
+  *dt = cfl_dt(m, *cfl);
+}
+
+static void normal_dt(const char* who, real_t* dt, const real_t* vn, const Box& vg, real_t vconst, const real_t* pxp,
+                      const real_t* pyp, const real_t* pzp, const Box& gp, const real_t* pxm, const real_t* pym,
+                      const real_t* pzm, const Box& gm, const Box& ib, real_t dx, real_t dy, real_t dz, real_t cfl) {
+  double m = -1.0;
+  if (!ib.empty()) {
+    if (!ok(who, ib, {{gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}})) return;
+    if (vn && !ok(who, ib, {{vg, "vel_n"}})) return;
+    real_t h = dx > dy ? dx : dy;
+    h = h > dz ? h : dz;
+    NormalCfl op{vn ? view(vn, vg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0},
+                 view(pxp, gp), view(pyp, gp), view(pzp, gp), view(pxm, gm), view(pym, gm), view(pzm, gm),
+                 vconst, h * h, real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz};
+    if (!reduce::box_max(who, ib, op, &m)) return;
+  }
+  *dt = cfl_dt(m, cfl);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestablenormalveldt_(real_t* dt, const real_t* vn, LSMHIP_BOXP(vg), const real_t* pxp,
+                                                  const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
+                                                  const real_t* pxm, const real_t* pym, const real_t* pzm,
+                                                  LSMHIP_BOXP(gm), LSMHIP_BOXP(ib), const real_t* dx,
+                                                  const real_t* dy, const real_t* dz, const real_t* cfl) {
+  normal_dt("lsm3dcomputestablenormalveldt_", dt, vn, LSMHIP_BOX(vg), real_t(0), pxp, pyp, pzp, LSMHIP_BOX(gp),
+            pxm, pym, pzm, LSMHIP_BOX(gm), LSMHIP_BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldt_(real_t* dt, const real_t* vel_n, const real_t* pxp,
+                                                       const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
+                                                       const real_t* pxm, const real_t* pym, const real_t* pzm,
+                                                       LSMHIP_BOXP(gm), LSMHIP_BOXP(ib), const real_t* dx,
+                                                       const real_t* dy, const real_t* dz, const real_t* cfl) {
+  Box none{0, -1, 0, -1, 0, -1};
+  normal_dt("lsm3dcomputestableconstnormalveldt_", dt, nullptr, none, real_t(fabs(*vel_n)), pxp, pyp, pzp,
+            LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), LSMHIP_BOX(ib), *dx, *dy, *dz, *cfl);
+}
+
+// ----- integrals -----------------------------------------------------------
+static double volume(const char* who, const real_t* F, const Box& fg, const real_t* phi, const Box& pg,
+                     const Box& ib, real_t dx, real_t dy, real_t dz, real_t eps, int sign, bool* good) {
+  *good = false;
+  if (ib.empty()) {
+    *good = true;
+    return 0.0;
+  }
+  if (!ok(who, ib, {{pg, "phi"}})) return 0.0;
+  if (F && !ok(who, ib, {{fg, "F"}})) return 0.0;
+  VolumeTerm op{view(phi, pg), F ? view(F, fg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, eps, dx * dy * dz,
+                sign};
+  double s = 0;
+  *good = reduce::box_sum(who, ib, op, &s);
+  return s;
+}
+
+static double surface(const char* who, const real_t* F, const Box& fg, const real_t* phi, const Box& pg,
+                      const real_t* px, const real_t* py, const real_t* pz, const Box& g, const Box& ib, real_t dx,
+                      real_t dy, real_t dz, real_t eps, bool* good) {
+  *good = false;
+  if (ib.empty()) {
+    *good = true;
+    return 0.0;
+  }
+  if (!ok(who, ib, {{pg, "phi"}, {g, "grad_phi"}})) return 0.0;
+  if (F && !ok(who, ib, {{fg, "F"}})) return 0.0;
+  SurfaceTerm op{view(phi, pg), F ? view(F, fg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, view(px, g),
+                 view(py, g), view(pz, g), eps, real_t(1.0) / eps, dx * dy * dz};
+  double s = 0;
+  *good = reduce::box_sum(who, ib, op, &s);
+  return s;
+}
+
+LSMHIP_EXPORT void lsm3dvolumeintegralphilessthanzero_(real_t* out, const real_t* F, LSMHIP_BOXP(fg),
+                                                       const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
+                                                       const real_t* dx, const real_t* dy, const real_t* dz,
+                                                       const real_t* eps) {
+  bool good;
+  double s = volume("lsm3dvolumeintegralphilessthanzero_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib),
+                    *dx, *dy, *dz, *eps, -1, &good);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvolumeintegralphigreaterthanzero_(real_t* out, const real_t* F, LSMHIP_BOXP(fg),
+                                                          const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
+                                                          const real_t* dx, const real_t* dy, const real_t* dz,
+                                                          const real_t* eps) {
+  bool good;
+  double s = volume("lsm3dvolumeintegralphigreaterthanzero_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg),
+                    LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, +1, &good);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceintegral_(real_t* out, const real_t* F, LSMHIP_BOXP(fg), const real_t* phi,
+                                         LSMHIP_BOXP(pg), const real_t* px, const real_t* py, const real_t* pz,
+                                         LSMHIP_BOXP(g), LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                         const real_t* dz, const real_t* eps) {
+  bool good;
+  double s = surface("lsm3dsurfaceintegral_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g),
+                     LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, &good);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvolumeregionphilessthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
+                                                     LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                                     const real_t* dz, const real_t* eps) {
+  bool good;
+  Box none{0, -1, 0, -1, 0, -1};
+  double s = volume("lsm3dvolumeregionphilessthanzero_", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), *dx,
+                    *dy, *dz, *eps, -1, &good);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvolumeregionphigreaterthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
+                                                        LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                                        const real_t* dz, const real_t* eps) {
+  bool good;
+  Box none{0, -1, 0, -1, 0, -1};
+  double s = volume("lsm3dvolumeregionphigreaterthanzero_", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), *dx,
+                    *dy, *dz, *eps, +1, &good);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceareazerolevelset_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg), const real_t* px,
+                                                 const real_t* py, const real_t* pz, LSMHIP_BOXP(g), LSMHIP_BOXP(ib),
+                                                 const real_t* dx, const real_t* dy, const real_t* dz,
+                                                 const real_t* eps) {
+  bool good;
+  Box none{0, -1, 0, -1, 0, -1};
+  double s = surface("lsm3dsurfaceareazerolevelset_", nullptr, none, phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g),
+                     LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, &good);
+  if (good) *out = real_t(s);
+}
+
+// FP64-accumulated volume of {phi < 0} returned in double whatever the storage
+// precision (used by the FP32 path for its interface-volume error).
+LSMHIP_EXPORT int lsmhip_volume_phi_lt0_f64(double* out, const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
+                                            const double* dx, const double* dy, const double* dz,
+                                            const double* eps) {
+  bool good;
+  Box none{0, -1, 0, -1, 0, -1};
+  double s = volume("lsmhip_volume_phi_lt0_f64", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), real_t(*dx),
+                    real_t(*dy), real_t(*dz), real_t(*eps), -1, &good);
+  if (good) *out = s;
+  return good ? 0 : -1;
+}
+
+// Smoothed Heaviside and delta (src/toolbox/lsm_calculus_toolbox.f:25-92); host scalars.
+LSMHIP_EXPORT void lsmheaviside_(real_t* H, const real_t* x, const real_t* epsilon) {
+  const real_t e = *epsilon, v = *x;
+  if (v < -e) *H = real_t(0);
+  else if (v > e) *H = real_t(1);
+  else {
+    const real_t ie = real_t(1.0) / e;
+    *H = real_t(0.5) * (real_t(1) + v * ie + real_t(kOneOverPi) * std::sin(real_t(kPi) * v * ie));
+  }
+}
+LSMHIP_EXPORT void lsmdeltafunction_(real_t* delta, const real_t* x, const real_t* epsilon) {
+  const real_t e = *epsilon, v = *x;
+  if (fabs(v) > e) *delta = real_t(0);
+  else {
+    const real_t ie = real_t(1.0) / e;
+    *delta = real_t(0.5) * ie * (real_t(1) + std::cos(real_t(kPi) * v * ie));
+  }
+}

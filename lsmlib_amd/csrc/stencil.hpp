@@ -1,0 +1,222 @@
+// stencil.hpp -- per-cell formulas of the level-set path, written so that the
+// FP64 build reproduces the reference's rounding: same operations, same
+// association order, no FMA contraction (the library is compiled with
+// -ffp-contract=off in double precision).  Shared by the drop-in kernels, the
+// narrow-band kernels and the FP64 fused stage kernels.
+#pragma once
+
+#include "lsmhip_core.hpp"
+
+namespace lsmhip {
+namespace stencil {
+
+// One HJ-WENO5 reconstruction from five scaled differences, in the order of
+// src/toolbox/lsm_spatial_derivatives3d.f.in:1296-1323.
+template <class T>
+__device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
+  const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0), eleven_sixths = T(11.0 / 6.0);
+  const T one_sixth = T(1.0 / 6.0), five_sixths = T(5.0 / 6.0);
+  const T thirteen_twelfths = T(13.0 / 12.0), one_fourth = T(0.25);
+  T m = v1 * v1;
+  m = v2 * v2 > m ? v2 * v2 : m;
+  m = v3 * v3 > m ? v3 * v3 : m;
+  m = v4 * v4 > m ? v4 * v4 : m;
+  m = v5 * v5 > m ? v5 * v5 : m;
+  const T eps = T(1e-6) * m + T(1.e-36);
+  const T p1 = one_third * v1 - seven_sixths * v2 + eleven_sixths * v3;
+  const T p2 = -(one_sixth * v2) + five_sixths * v3 + one_third * v4;
+  const T p3 = one_third * v3 + five_sixths * v4 - one_sixth * v5;
+  const T S1 = thirteen_twelfths * sq(v1 - T(2) * v2 + v3) + one_fourth * sq(v1 - T(4) * v2 + T(3) * v3);
+  const T S2 = thirteen_twelfths * sq(v2 - T(2) * v3 + v4) + one_fourth * sq(v2 - v4);
+  const T S3 = thirteen_twelfths * sq(v3 - T(2) * v4 + v5) + one_fourth * sq(T(3) * v3 - T(4) * v4 + v5);
+  T a1 = T(0.1) / sq(S1 + eps);
+  T a2 = T(0.6) / sq(S2 + eps);
+  T a3 = T(0.3) / sq(S3 + eps);
+  const T inv = T(1.0) / (a1 + a2 + a3);
+  a1 = a1 * inv;
+  a2 = a2 * inv;
+  a3 = a3 * inv;
+  return a1 * p1 + a2 * p2 + a3 * p3;
+}
+
+// Undivided differences from seven samples f[s+3] = phi(offset s) along an axis:
+// D1(s) = f(s) - f(s-1) (face s-1/2 stored at s), D2(s) = -(D1(s) - D1(s+1)),
+// D3(s) = D2(s) - D2(s-1)  (lsmComputeDn, lsm_spatial_derivatives3d.f.in:49-175).
+#define LSM_D1(s) (f[(s) + 3] - f[(s) + 2])
+
+template <class T>
+__device__ __forceinline__ T eno2_plus(const T* d1, const T* d2) {  // d1[s+2], d2[s+2]
+  return fabs(d2[2]) < fabs(d2[3]) ? d1[3] - T(0.5) * d2[2] : d1[3] - T(0.5) * d2[3];
+}
+template <class T>
+__device__ __forceinline__ T eno2_minus(const T* d1, const T* d2) {
+  return fabs(d2[1]) < fabs(d2[2]) ? d1[2] + T(0.5) * d2[1] : d1[2] + T(0.5) * d2[2];
+}
+// ENO3, lsm_spatial_derivatives3d.f.in:837-903 (one rounding per accumulation).
+template <class T>
+__device__ __forceinline__ T eno3_plus(const T* d1, const T* d2, const T* d3) {
+  const T third = T(1.0 / 3.0), sixth = T(1.0 / 6.0);
+  T r = d1[3];
+  if (fabs(d2[2]) < fabs(d2[3])) {
+    r = r - T(0.5) * d2[2];
+    r = fabs(d3[2]) < fabs(d3[3]) ? r - sixth * d3[2] : r - sixth * d3[3];
+  } else {
+    r = r - T(0.5) * d2[3];
+    r = fabs(d3[3]) < fabs(d3[4]) ? r + third * d3[3] : r + third * d3[4];
+  }
+  return r;
+}
+template <class T>
+__device__ __forceinline__ T eno3_minus(const T* d1, const T* d2, const T* d3) {
+  const T third = T(1.0 / 3.0), sixth = T(1.0 / 6.0);
+  T r = d1[2];
+  if (fabs(d2[1]) < fabs(d2[2])) {
+    r = r + T(0.5) * d2[1];
+    r = fabs(d3[1]) < fabs(d3[2]) ? r + third * d3[1] : r + third * d3[2];
+  } else {
+    r = r + T(0.5) * d2[2];
+    r = fabs(d3[2]) < fabs(d3[3]) ? r - sixth * d3[2] : r - sixth * d3[3];
+  }
+  return r;
+}
+
+// Differences indexed [s+2] for s = -2..3 (d1), -2..2 (d2), -1..2 (d3 at [s+2]).
+template <class T>
+__device__ __forceinline__ void differences(const T f[7], T d1[6], T d2[5], T d3[5]) {
+#pragma unroll
+  for (int s = -2; s <= 3; s++) d1[s + 2] = LSM_D1(s);
+#pragma unroll
+  for (int s = -2; s <= 2; s++) d2[s + 2] = -(d1[s + 2] - d1[s + 3]);
+#pragma unroll
+  for (int s = -1; s <= 2; s++) d3[s + 2] = d2[s + 2] - d2[s + 1];
+}
+
+// Plus and minus one-sided derivatives of order 1, 2, 3 (ENO) or 5 (WENO).
+template <int ORDER, class T>
+__device__ __forceinline__ void plus_minus(const T f[7], T inv, T& p, T& m) {
+  if (ORDER == 1) {
+    p = LSM_D1(1) * inv;
+    m = LSM_D1(0) * inv;
+  } else if (ORDER == 2) {
+    T d1[6], d2[5];
+#pragma unroll
+    for (int s = -1; s <= 2; s++) d1[s + 2] = LSM_D1(s);
+#pragma unroll
+    for (int s = -1; s <= 1; s++) d2[s + 2] = -(d1[s + 2] - d1[s + 3]);
+    p = eno2_plus(d1, d2) * inv;
+    m = eno2_minus(d1, d2) * inv;
+  } else if (ORDER == 3) {
+    T d1[6], d2[5], d3[5];
+    differences(f, d1, d2, d3);
+    p = eno3_plus(d1, d2, d3) * inv;
+    m = eno3_minus(d1, d2, d3) * inv;
+  } else {
+    T v[6];
+#pragma unroll
+    for (int s = -2; s <= 3; s++) v[s + 2] = LSM_D1(s) * inv;
+    p = weno5(v[5], v[4], v[3], v[2], v[1]);
+    m = weno5(v[0], v[1], v[2], v[3], v[4]);
+  }
+}
+
+// Velocity-upwinded derivative: v > 0 takes the minus (backward) side.
+template <int ORDER, class T>
+__device__ __forceinline__ T upwind(const T f[7], T inv, bool vpos) {
+  if (ORDER == 1) return (vpos ? LSM_D1(0) : LSM_D1(1)) * inv;
+  if (ORDER == 2) {
+    T d1[6], d2[5];
+#pragma unroll
+    for (int s = -1; s <= 2; s++) d1[s + 2] = LSM_D1(s);
+#pragma unroll
+    for (int s = -1; s <= 1; s++) d2[s + 2] = -(d1[s + 2] - d1[s + 3]);
+    return (vpos ? eno2_minus(d1, d2) : eno2_plus(d1, d2)) * inv;
+  }
+  if (ORDER == 3) {
+    T d1[6], d2[5], d3[5];
+    differences(f, d1, d2, d3);
+    return (vpos ? eno3_minus(d1, d2, d3) : eno3_plus(d1, d2, d3)) * inv;
+  }
+  if (vpos) return weno5(LSM_D1(-2) * inv, LSM_D1(-1) * inv, LSM_D1(0) * inv, LSM_D1(1) * inv, LSM_D1(2) * inv);
+  return weno5(LSM_D1(3) * inv, LSM_D1(2) * inv, LSM_D1(1) * inv, LSM_D1(0) * inv, LSM_D1(-1) * inv);
+}
+#undef LSM_D1
+
+// |grad phi| with per-component and result thresholds
+// (lsm_spatial_derivatives3d.f.in:3622-3650 uses 1e-8; the local twin uses zero_tol).
+template <class T>
+__device__ __forceinline__ T grad_magnitude(T x, T y, T z, T tol) {
+  T sx = x * x, sy = y * y, sz = z * z;
+  if (sx < tol) sx = T(0);
+  if (sy < tol) sy = T(0);
+  if (sz < tol) sz = T(0);
+  const T t = sqrt(sx + sy + sz);
+  return t < tol ? T(0) : t;
+}
+
+// Godunov |grad phi|^2 for normal speed of sign vpos
+// (lsm_level_set_evolution3d.f.in:210-224).
+template <class T>
+__device__ __forceinline__ T godunov_sq(bool vpos, T xp, T yp, T zp, T xm, T ym, T zm) {
+  const T z0 = T(0);
+  if (vpos)
+    return maxr(sq(maxr(xm, z0)), sq(minr(xp, z0))) + maxr(sq(maxr(ym, z0)), sq(minr(yp, z0))) +
+           maxr(sq(maxr(zm, z0)), sq(minr(zp, z0)));
+  return maxr(sq(minr(xm, z0)), sq(maxr(xp, z0))) + maxr(sq(minr(ym, z0)), sq(maxr(yp, z0))) +
+         maxr(sq(minr(zm, z0)), sq(maxr(zp, z0)));
+}
+
+// Reinitialization RHS of one cell, lsm_reinitialization3d.f.in:167-199 (and :251 for phi0).
+template <class T>
+__device__ __forceinline__ T reinit_rhs(T s, bool use_phi0, T xp, T yp, T zp, T xm, T ym, T zm, T dx_sq) {
+  const T gp[3] = {xp, yp, zp}, gm[3] = {xm, ym, zm};
+  T star[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T p, m;
+    if (s > T(0)) {
+      p = maxr(-gp[d], T(0));
+      m = maxr(gm[d], T(0));
+    } else {
+      p = maxr(gp[d], T(0));
+      m = maxr(-gm[d], T(0));
+    }
+    star[d] = maxr(p, m);
+  }
+  if (fabs(s) >= kZeroTol) {
+    const T n2 = star[0] * star[0] + star[1] * star[1] + star[2] * star[2];
+    const T sgn = use_phi0 ? s / sqrt(s * s + dx_sq) : s / sqrt(s * s + n2 * dx_sq);
+    return sgn * (T(1) - sqrt(n2));
+  }
+  return T(0);
+}
+
+// Curvature numerator shared by lsm_level_set_evolution3d.f.in:433-451 and
+// lsm_curvature3d.f.in:103-120.
+template <class T>
+__device__ __forceinline__ T curv_num(T xx, T xy, T xz, T yy, T yz, T zz, T x, T y, T z) {
+  return xx * y * y + yy * x * x - T(2) * xy * x * y + xx * z * z + zz * x * x - T(2) * xz * x * z + yy * z * z +
+         zz * y * y - T(2) * yz * y * z;
+}
+
+// Per-axis Godunov choice for external + normal motion,
+// lsm_level_set_evolution3d.f.in:629-693.
+template <class T>
+__device__ __forceinline__ T ext_normal_pick(T u, T vn, T pp, T pm) {
+  const T hp = u + vn * pp, hm = u + vn * pm;
+  if (hp > T(0)) return hm > T(0) ? pm : -(u / vn);
+  if (hm < T(0)) return pp;
+  return fabs(hp) > fabs(hm) ? pp : pm;
+}
+
+// Normal-motion CFL quantity |V| |grad|_1-weighted / |grad|, lsm_math_utils3d.f.in:413-429.
+template <class T>
+__device__ __forceinline__ T normal_h_over_dx(T absv, T xp, T yp, T zp, T xm, T ym, T zm, T mdx2, T ix, T iy, T iz,
+                                              T* norm_out = nullptr) {
+  const T x = maxr(fabs(xp), fabs(xm)), y = maxr(fabs(yp), fabs(ym)), z = maxr(fabs(zp), fabs(zm));
+  const T n = sqrt(x * x + y * y + z * z + mdx2);
+  if (norm_out) *norm_out = n;
+  return absv / n * (x * ix + y * iy + z * iz);
+}
+
+}  // namespace stencil
+}  // namespace lsmhip
