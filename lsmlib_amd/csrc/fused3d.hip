@@ -1,0 +1,311 @@
+// fused3d.hip -- one TVD Runge-Kutta stage of the level-set equation in a
+// single kernel: HJ-WENO5 one-sided derivatives, the Hamiltonian (normal
+// motion with Godunov selection, velocity-upwinded advection, or the
+// reinitialization RHS) and the RK combination, with no derivative, RHS or
+// velocity array written to memory.
+//
+// Memory layout and traffic: phi_stage, phi_n and the output share one ghost
+// box.  A block owns a TX x TY tile of the (i,j) plane and marches along k
+// through a chunk of KC planes, keeping a ring of 7 planes (the WENO5 reach
+// is +-3) of phi_stage with their x/y halo in LDS; each plane is read from
+// global memory once per block.  Neighbouring tiles share halos through L2:
+// blocks are numbered so that each XCD (blocks are dealt round-robin to the 8
+// XCDs) processes one contiguous range of tiles.
+//
+// The per-cell arithmetic is that of the drop-in chain (stencil.hpp), so in
+// the FP64 build a stage is bitwise equal to
+//   HJ_WENO5 / UPWIND_HJ_WENO5 -> zero RHS -> add term -> TVD_RK3_STAGEn
+// of the reference on the same inputs.
+#include "lsmhip_core.hpp"
+#include "stencil.hpp"
+
+using namespace lsmhip;
+
+namespace {
+
+constexpr int TX = 64, TY = 8, H = 3;
+constexpr int PX = TX + 2 * H, PY = TY + 2 * H, PLANE = PX * PY;
+constexpr int RING = 2 * H + 1;
+
+enum RhsKind { kNormalConst = 0, kNormalField = 1, kAdvectField = 2, kAdvectDeform = 3, kReinit = 4 };
+
+struct FusedArgs {
+  const real_t* us;  // field differentiated this stage (ghosts filled)
+  const real_t* uc;  // phi at the start of the step (stages 2 and 3)
+  real_t* out;
+  Box gb, fb;
+  int sy, sz;
+  real_t ix, iy, iz, dt;
+  int rhs;
+  real_t vn;                          // constant normal speed
+  const real_t *vnf, *vx, *vy, *vz;   // per-cell fields (gb addressing)
+  const real_t* phi0;                 // reinitialization sign source
+  int use_phi0;
+  real_t dx_sq;                       // max(dx,dy,dz)^2 for the smoothed sign
+  const real_t *ta[3], *tb[3];        // deformation tables A = sin^2(pi x), B = sin(2 pi x), per axis
+  real_t tfac;                        // cos(pi t / T) of the deformation field
+  int nti, ntj, ntk, kc;              // tiles in i, j, chunks in k, planes per chunk
+};
+
+// Deformation velocity u = 2 A(x)B(y)B(z)c, v = -B(x)A(y)B(z)c, w = -B(x)B(y)A(z)c
+// from per-axis tables indexed from the ghost-box origin.
+__device__ __forceinline__ void deform_velocity(const real_t* const ta[3], const real_t* const tb[3], int ii, int jj,
+                                                int kk, real_t c, real_t& u, real_t& v, real_t& w) {
+  u = real_t(2) * ta[0][ii] * tb[1][jj] * tb[2][kk] * c;
+  v = -(tb[0][ii] * ta[1][jj] * tb[2][kk] * c);
+  w = -(tb[0][ii] * tb[1][jj] * ta[2][kk] * c);
+}
+
+template <int STAGE>
+__device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, real_t r) {
+  if (STAGE == 1) return uc + dt * r;
+  if (STAGE == 2) return real_t(0.75) * uc + real_t(0.25) * (us + dt * r);
+  if (STAGE == 3) return real_t(1.0 / 3.0) * uc + real_t(2.0 / 3.0) * (us + dt * r);
+  return real_t(0.5) * (uc + us + dt * r);  // TVD-RK2 stage 2
+}
+
+template <int RHS, int STAGE>
+__global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  real_t* ring = reinterpret_cast<real_t*>(smem_raw);
+  const int tid = threadIdx.y * TX + threadIdx.x;
+
+  // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a contiguous tile range.
+  const int nt = a.nti * a.ntj * a.ntk;
+  const int b = blockIdx.x;
+  const int per = (nt + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= nt) return;
+  const int ti = t % a.nti, tj = (t / a.nti) % a.ntj, tk = t / (a.nti * a.ntj);
+  const int i0 = a.fb.ilo + ti * TX, j0 = a.fb.jlo + tj * TY;
+  const int kb = a.fb.klo + tk * a.kc;
+  const int ke = min(kb + a.kc - 1, a.fb.khi);
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
+
+  const Box gb = a.gb;
+  auto load_plane = [&](int kp) {
+    real_t* dst = ring + ((kp - kb + 3 * RING) % RING) * PLANE;
+    for (int e = tid; e < PLANE; e += TX * TY) {
+      const int gi = i0 - H + e % PX, gj = j0 - H + e / PX;
+      if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi && kp >= gb.klo && kp <= gb.khi)
+        dst[e] = a.us[(gi - gb.ilo) + a.sy * (gj - gb.jlo) + a.sz * (kp - gb.klo)];
+    }
+  };
+  for (int kp = kb - H; kp < kb + H; kp++) load_plane(kp);
+
+  const int lx = threadIdx.x + H, ly = threadIdx.y + H;
+  for (int k = kb; k <= ke; k++) {
+    load_plane(k + H);
+    __syncthreads();
+    if (active) {
+      const real_t* pc = ring + ((k - kb + 3 * RING) % RING) * PLANE;
+      real_t fx[7], fy[7], fz[7];
+#pragma unroll
+      for (int s = -H; s <= H; s++) {
+        fx[s + 3] = pc[ly * PX + lx + s];
+        fy[s + 3] = pc[(ly + s) * PX + lx];
+        fz[s + 3] = ring[((k + s - kb + 3 * RING) % RING) * PLANE + ly * PX + lx];
+      }
+      const int off = (i - gb.ilo) + a.sy * (j - gb.jlo) + a.sz * (k - gb.klo);
+      real_t r;
+      if (RHS == kAdvectField || RHS == kAdvectDeform) {
+        real_t u, v, w;
+        if (RHS == kAdvectField) {
+          u = a.vx[off];
+          v = a.vy[off];
+          w = a.vz[off];
+        } else {
+          deform_velocity(a.ta, a.tb, i - gb.ilo, j - gb.jlo, k - gb.klo, a.tfac, u, v, w);
+        }
+        const real_t px = fabs(u) < kZeroTol ? real_t(0) : stencil::upwind<5>(fx, a.ix, u > real_t(0));
+        const real_t py = fabs(v) < kZeroTol ? real_t(0) : stencil::upwind<5>(fy, a.iy, v > real_t(0));
+        const real_t pz = fabs(w) < kZeroTol ? real_t(0) : stencil::upwind<5>(fz, a.iz, w > real_t(0));
+        r = real_t(0) - (u * px + v * py + w * pz);
+      } else {
+        real_t xp, xm, yp, ym, zp, zm;
+        stencil::plus_minus<5>(fx, a.ix, xp, xm);
+        stencil::plus_minus<5>(fy, a.iy, yp, ym);
+        stencil::plus_minus<5>(fz, a.iz, zp, zm);
+        if (RHS == kReinit) {
+          const real_t s = a.use_phi0 ? a.phi0[off] : fx[3];
+          r = stencil::reinit_rhs(s, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
+        } else {
+          const real_t v = RHS == kNormalField ? a.vnf[off] : a.vn;
+          r = real_t(0);
+          if (fabs(v) >= kZeroTol) r = r - v * sqrt(stencil::godunov_sq(v > real_t(0), xp, yp, zp, xm, ym, zm));
+        }
+      }
+      const real_t uc = STAGE == 1 ? fx[3] : a.uc[off];
+      a.out[off] = rk_combine<STAGE>(uc, fx[3], a.dt, r);
+    }
+    __syncthreads();
+  }
+}
+
+template <int RHS>
+bool launch_rhs(int stage, const FusedArgs& a, int nblocks, size_t lds) {
+  switch (stage) {
+    case 1: hipLaunchKernelGGL((fused_stage_kernel<RHS, 1>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
+    case 2: hipLaunchKernelGGL((fused_stage_kernel<RHS, 2>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
+    case 3: hipLaunchKernelGGL((fused_stage_kernel<RHS, 3>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
+    case 22: hipLaunchKernelGGL((fused_stage_kernel<RHS, 22>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
+    default: return false;
+  }
+  return true;
+}
+
+int g_kchunk = 64;
+
+int run(const char* who, FusedArgs a, int stage) {
+  if (a.fb.empty()) return 0;
+  if (!check_box(a.gb, a.fb.widen(H, H, H), who, "phi") || !check_size(a.gb, who)) return -1;
+  if (stage == 1 && a.uc == nullptr) a.uc = a.us;
+  if (stage != 1 && a.uc == nullptr) {
+    set_error(kErrArg, "%s: stage %d needs phi at the start of the step", who, stage);
+    return -1;
+  }
+  a.sy = a.gb.nx();
+  a.sz = a.gb.nx() * a.gb.ny();
+  a.kc = g_kchunk;
+  a.nti = (a.fb.nx() + TX - 1) / TX;
+  a.ntj = (a.fb.ny() + TY - 1) / TY;
+  a.ntk = (a.fb.nz() + a.kc - 1) / a.kc;
+  const long long nt = (long long)a.nti * a.ntj * a.ntk;
+  const int per = (int)((nt + 7) / 8);
+  const int nblocks = 8 * per;  // surplus blocks of the last XCD range exit at once
+  const size_t lds = sizeof(real_t) * PLANE * RING;
+  bool ok = false;
+  switch (a.rhs) {
+    case kNormalConst: ok = launch_rhs<kNormalConst>(stage, a, nblocks, lds); break;
+    case kNormalField: ok = launch_rhs<kNormalField>(stage, a, nblocks, lds); break;
+    case kAdvectField: ok = launch_rhs<kAdvectField>(stage, a, nblocks, lds); break;
+    case kAdvectDeform: ok = launch_rhs<kAdvectDeform>(stage, a, nblocks, lds); break;
+    case kReinit: ok = launch_rhs<kReinit>(stage, a, nblocks, lds); break;
+  }
+  if (!ok) {
+    set_error(kErrArg, "%s: stage must be 1, 2, 3 (TVD-RK3) or 22 (TVD-RK2 stage 2), got %d", who, stage);
+    return -1;
+  }
+  finish(who);
+  return 0;
+}
+
+FusedArgs base(real_t* out, const real_t* us, const real_t* uc, const Box& gb, const Box& fb, real_t dt, real_t dx,
+               real_t dy, real_t dz) {
+  FusedArgs a{};
+  a.us = us;
+  a.uc = uc;
+  a.out = out;
+  a.gb = gb;
+  a.fb = fb;
+  a.dt = dt;
+  a.ix = real_t(1.0) / dx;
+  a.iy = real_t(1.0) / dy;
+  a.iz = real_t(1.0) / dz;
+  return a;
+}
+
+__global__ void __launch_bounds__(256) deform_fill_kernel(Box b, Box gb, real_t* vx, real_t* vy, real_t* vz,
+                                                          const real_t* ta0, const real_t* ta1, const real_t* ta2,
+                                                          const real_t* tb0, const real_t* tb1, const real_t* tb2,
+                                                          real_t c) {
+  LSMHIP_CELL(b);
+  const real_t* ta[3] = {ta0, ta1, ta2};
+  const real_t* tb[3] = {tb0, tb1, tb2};
+  real_t u, v, w;
+  deform_velocity(ta, tb, i - gb.ilo, j - gb.jlo, k - gb.klo, c, u, v, w);
+  const int off = (i - gb.ilo) + gb.nx() * ((j - gb.jlo) + gb.ny() * (k - gb.klo));
+  vx[off] = u;
+  vy[off] = v;
+  vz[off] = w;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fused stage entry points (no counterpart in the reference; the per-cell
+// arithmetic is that of the drop-in chain named in each comment).  All arrays
+// are device arrays on the ghost box gb; the stage writes phi_out on fb only.
+// stage: 1, 2, 3 = TVD-RK3 stages (lsm3dTVDRK3Stage1/2/3), 22 = TVD-RK2 stage 2.
+// phi_n may be null for stage 1.  Returns 0, or -1 with lsmhip_get_last_error set.
+// ---------------------------------------------------------------------------
+
+// HJ_WENO5 -> ADD_[CONST_]NORMAL_VEL_TERM -> RK stage.  vn_field null: constant speed vn.
+LSMHIP_EXPORT int lsmhip_weno5_stage_normal(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
+                                            LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, real_t vn,
+                                            const real_t* vn_field, real_t dt, real_t dx, real_t dy, real_t dz) {
+  FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
+  a.rhs = vn_field ? kNormalField : kNormalConst;
+  a.vn = vn;
+  a.vnf = vn_field;
+  if (!vn_field && !(fabs(vn) >= kZeroTol)) a.vn = real_t(0);
+  return run("lsmhip_weno5_stage_normal", a, stage);
+}
+
+// UPWIND_HJ_WENO5 -> ADD_ADVECTION_TERM -> RK stage, velocity arrays on gb.
+LSMHIP_EXPORT int lsmhip_weno5_stage_advect(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
+                                            LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, const real_t* vx,
+                                            const real_t* vy, const real_t* vz, real_t dt, real_t dx, real_t dy,
+                                            real_t dz) {
+  FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
+  a.rhs = kAdvectField;
+  a.vx = vx;
+  a.vy = vy;
+  a.vz = vz;
+  return run("lsmhip_weno5_stage_advect", a, stage);
+}
+
+// As lsmhip_weno5_stage_advect with the divergence-free deformation velocity
+//   u = 2 sin^2(pi x) sin(2 pi y) sin(2 pi z) c,  v = -sin(2 pi x) sin^2(pi y) sin(2 pi z) c,
+//   w = -sin(2 pi x) sin(2 pi y) sin^2(pi z) c
+// evaluated in the kernel from per-axis tables (device arrays indexed from the
+// ghost-box origin: tabs = {A_x, A_y, A_z, B_x, B_y, B_z}, A = sin^2(pi x),
+// B = sin(2 pi x)); c is the time factor.  No velocity array is read.
+LSMHIP_EXPORT int lsmhip_weno5_stage_advect_deform(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
+                                                   LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage,
+                                                   const real_t* const* tabs, real_t c, real_t dt, real_t dx,
+                                                   real_t dy, real_t dz) {
+  FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
+  a.rhs = kAdvectDeform;
+  for (int d = 0; d < 3; d++) {
+    a.ta[d] = tabs[d];
+    a.tb[d] = tabs[3 + d];
+  }
+  a.tfac = c;
+  return run("lsmhip_weno5_stage_advect_deform", a, stage);
+}
+
+// HJ_WENO5 -> COMPUTE_REINITIALIZATION_EQN_RHS -> RK stage.
+LSMHIP_EXPORT int lsmhip_weno5_stage_reinit(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
+                                            LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, const real_t* phi0,
+                                            int use_phi0_for_sgn, real_t dt, real_t dx, real_t dy, real_t dz) {
+  FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
+  a.rhs = kReinit;
+  a.phi0 = phi0;
+  a.use_phi0 = use_phi0_for_sgn == 1;
+  if (a.use_phi0 && !phi0) {
+    set_error(kErrArg, "lsmhip_weno5_stage_reinit: use_phi0_for_sgn set without phi0");
+    return -1;
+  }
+  real_t h = dx > dy ? dx : dy;
+  h = h > dz ? h : dz;
+  a.dx_sq = h * h;
+  return run("lsmhip_weno5_stage_reinit", a, stage);
+}
+
+// Writes the deformation velocity of lsmhip_weno5_stage_advect_deform into
+// three arrays on gb over box b (for the drop-in chain and checks).
+LSMHIP_EXPORT int lsmhip_deform_velocity_fill(real_t* vx, real_t* vy, real_t* vz, LSMHIP_BOXP(gb), LSMHIP_BOXP(b),
+                                              const real_t* const* tabs, real_t c) {
+  const char* who = "lsmhip_deform_velocity_fill";
+  Box gb = LSMHIP_BOX(gb), bb = LSMHIP_BOX(b);
+  if (bb.empty()) return 0;
+  if (!check_box(gb, bb, who, "vel") || !check_size(gb, who)) return -1;
+  hipLaunchKernelGGL(deform_fill_kernel, grid_for(bb), block2d(), 0, stream(), bb, gb, vx, vy, vz, tabs[0], tabs[1],
+                     tabs[2], tabs[3], tabs[4], tabs[5], c);
+  finish(who);
+  return 0;
+}
+
+LSMHIP_EXPORT void lsmhip_set_fused_kchunk(int kc) { g_kchunk = kc < 1 ? 1 : kc; }

@@ -1,0 +1,314 @@
+"""Host side of the time-stepping path: grids, seeded inputs and RK drivers.
+
+Arrays are torch tensors on the GPU (plumbing only: device memory and the
+stream); every operation on them is a call into the HIP library through its
+C-ABI.  A field on a grid with n = (nx, ny, nz) interior points and g ghost
+layers is stored column-major over the ghost box (-g .. n-1+g per axis), i.e.
+a contiguous tensor of shape (nz+2g, ny+2g, nx+2g).  Coordinates follow the
+reference's Grid (src/utils/lsm_grid.c:189-232, src/toolbox/lsm_initialization3d.c:142-144):
+dx = (x_hi - x_lo)/n and x(i) = x_lo + i*dx for index i.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from .lib import LsmHipError, load
+
+BC_KIND = {"linear": 0, "signed_linear": 1, "copy": 2, "neumann": 2, "periodic": 3}
+STAGE3 = (1, 2, 3)
+
+
+# ---------------------------------------------------------------------------
+# Seeded inputs: splitmix64 -> doubles in [0, 1)
+# ---------------------------------------------------------------------------
+class SplitMix64:
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+
+class Grid:
+    def __init__(self, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), ghost=3):
+        self.n = tuple(int(v) for v in n)
+        self.lo = tuple(float(v) for v in lo)
+        self.hi = tuple(float(v) for v in hi)
+        self.g = int(ghost)
+        self.dx = tuple((self.hi[d] - self.lo[d]) / self.n[d] for d in range(3))
+        g = self.g
+        self.gb = (-g, self.n[0] - 1 + g, -g, self.n[1] - 1 + g, -g, self.n[2] - 1 + g)
+        self.fb = (0, self.n[0] - 1, 0, self.n[1] - 1, 0, self.n[2] - 1)
+        self.shape = (self.n[2] + 2 * g, self.n[1] + 2 * g, self.n[0] + 2 * g)
+
+    @property
+    def cells(self) -> int:
+        return self.n[0] * self.n[1] * self.n[2]
+
+    def axis(self, d):
+        """Coordinates of the ghost-box index range along axis d (float64)."""
+        return self.lo[d] + self.dx[d] * np.arange(-self.g, self.n[d] + self.g, dtype=np.float64)
+
+    def mesh(self):
+        z, y, x = np.meshgrid(self.axis(2), self.axis(1), self.axis(0), indexing="ij")
+        return x, y, z
+
+    def interior(self, a):
+        g = self.g
+        return a[g:a.shape[0] - g, g:a.shape[1] - g, g:a.shape[2] - g]
+
+
+# ---------------------------------------------------------------------------
+# Initial level sets of the benchmark configurations
+# ---------------------------------------------------------------------------
+def phi_perturbed_sphere(grid: Grid, seed: int):
+    """(|x-c| - 0.5)(1 + 0.1 sin 3pi x sin 3pi y sin 3pi z), c ~ U[-0.1, 0.1]^3: not a distance function."""
+    r = SplitMix64(seed)
+    c = [r.uniform(-0.1, 0.1) for _ in range(3)]
+    x, y, z = grid.mesh()
+    d = np.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
+    return (d - 0.5) * (1 + 0.1 * np.sin(3 * np.pi * x) * np.sin(3 * np.pi * y) * np.sin(3 * np.pi * z)), c
+
+
+def phi_spheres_min(grid: Grid, seed: int, count: int, centre_range, radius_range, chunk=16):
+    """min over seeded spheres of (|x - c_l| - r_l)."""
+    r = SplitMix64(seed)
+    spheres = []
+    for _ in range(count):
+        c = [r.uniform(*centre_range) for _ in range(3)]
+        spheres.append((c, r.uniform(*radius_range)))
+    ax = [grid.axis(d) for d in range(3)]
+    out = np.full(grid.shape, np.inf)
+    for (c, rad) in spheres:
+        dx2 = (ax[0] - c[0]) ** 2
+        dy2 = (ax[1] - c[1]) ** 2
+        dz2 = (ax[2] - c[2]) ** 2
+        for k0 in range(0, grid.shape[0], chunk):
+            blk = np.sqrt(dz2[k0:k0 + chunk, None, None] + dy2[None, :, None] + dx2[None, None, :]) - rad
+            np.minimum(out[k0:k0 + chunk], blk, out=out[k0:k0 + chunk])
+    return out, spheres
+
+
+def phi_sphere(grid: Grid, centre, radius, chunk=32):
+    ax = [grid.axis(d) for d in range(3)]
+    out = np.empty(grid.shape)
+    dx2 = (ax[0] - centre[0]) ** 2
+    dy2 = (ax[1] - centre[1]) ** 2
+    dz2 = (ax[2] - centre[2]) ** 2
+    for k0 in range(0, grid.shape[0], chunk):
+        out[k0:k0 + chunk] = np.sqrt(dz2[k0:k0 + chunk, None, None] + dy2[None, :, None] + dx2[None, None, :]) - radius
+    return out
+
+
+def deformation_tables(grid: Grid):
+    """Per-axis A = sin^2(pi x), B = sin(2 pi x) over the ghost-box index range."""
+    A = [np.sin(np.pi * grid.axis(d)) ** 2 for d in range(3)]
+    B = [np.sin(2 * np.pi * grid.axis(d)) for d in range(3)]
+    return A + B
+
+
+# ---------------------------------------------------------------------------
+# Solver: RK3 steps through the fused stage kernels or the drop-in chain
+# ---------------------------------------------------------------------------
+def _box_args(b):
+    return [ctypes.byref(ctypes.c_int(v)) for v in b]
+
+
+class Solver:
+    """Level-set time stepping on one grid in one precision ("f64" or "f32")."""
+
+    def __init__(self, grid: Grid, precision="f64", bc="signed_linear", device="cuda"):
+        import torch
+
+        self.torch = torch
+        self.grid = grid
+        self.precision = precision
+        self.lib = load(precision)
+        self.dtype = torch.float64 if precision == "f64" else torch.float32
+        self.real = self.lib.real
+        self.device = device
+        self.bc = bc
+        self._keep = []
+        L = self.lib
+        P, I, R = ctypes.c_void_p, ctypes.c_int, self.real
+        box = [ctypes.POINTER(ctypes.c_int)] * 6
+        self._fill = L.fn("lsmhip_fill_ghosts", [P] + box + box + [I, I])
+        stage_head = [P, P, P] + box + box + [I]
+        self._normal = L.fn("lsmhip_weno5_stage_normal", stage_head + [R, P, R, R, R, R])
+        self._advect = L.fn("lsmhip_weno5_stage_advect", stage_head + [P, P, P, R, R, R, R])
+        self._deform = L.fn("lsmhip_weno5_stage_advect_deform", stage_head + [P, R, R, R, R, R])
+        self._reinit = L.fn("lsmhip_weno5_stage_reinit", stage_head + [P, I, R, R, R, R])
+        self._velfill = L.fn("lsmhip_deform_velocity_fill", [P, P, P] + box + box + [P, R])
+        self._vol = L.fn("lsmhip_volume_phi_lt0_f64",
+                         [P, P] + box + box + [ctypes.POINTER(ctypes.c_double)] * 4)
+        self._gbp = [ctypes.c_int(v) for v in grid.gb]
+        self._fbp = [ctypes.c_int(v) for v in grid.fb]
+        self._tabs = None
+
+    # -- memory ---------------------------------------------------------------
+    def field(self, values=None):
+        t = self.torch
+        if values is None:
+            return t.zeros(self.grid.shape, dtype=self.dtype, device=self.device)
+        return t.as_tensor(np.ascontiguousarray(values), dtype=self.dtype).to(self.device)
+
+    def host(self, a):
+        return a.detach().to("cpu", self.torch.float64).numpy()
+
+    def _gb(self):
+        return [ctypes.byref(v) for v in self._gbp]
+
+    def _fb(self):
+        return [ctypes.byref(v) for v in self._fbp]
+
+    def _ok(self, rc):
+        if rc != 0:
+            self.lib.check()
+            raise LsmHipError("fused entry point failed")
+
+    # -- ghost cells --------------------------------------------------------------
+    def fill_ghosts(self, phi, kind=None):
+        kind = BC_KIND[kind or self.bc]
+        self._ok(self._fill(phi.data_ptr(), *self._gb(), *self._fb(), 9, kind))
+
+    # -- fused stages ---------------------------------------------------------------
+    def stage_normal(self, out, us, uc, stage, dt, vn=1.0, vn_field=None):
+        g = self.grid
+        self._ok(self._normal(out.data_ptr(), us.data_ptr(), uc.data_ptr() if uc is not None else None,
+                              *self._gb(), *self._fb(), stage, vn,
+                              vn_field.data_ptr() if vn_field is not None else None, dt, *g.dx))
+
+    def stage_advect(self, out, us, uc, stage, dt, vel):
+        g = self.grid
+        self._ok(self._advect(out.data_ptr(), us.data_ptr(), uc.data_ptr() if uc is not None else None,
+                              *self._gb(), *self._fb(), stage, vel[0].data_ptr(), vel[1].data_ptr(),
+                              vel[2].data_ptr(), dt, *g.dx))
+
+    def deform_tables(self):
+        if self._tabs is None:
+            tabs = [self.torch.as_tensor(a, dtype=self.dtype).to(self.device) for a in deformation_tables(self.grid)]
+            arr = (ctypes.c_void_p * 6)(*[a.data_ptr() for a in tabs])
+            self._tabs = (tabs, arr)
+        return self._tabs[1]
+
+    def stage_deform(self, out, us, uc, stage, dt, c):
+        g = self.grid
+        self._ok(self._deform(out.data_ptr(), us.data_ptr(), uc.data_ptr() if uc is not None else None,
+                              *self._gb(), *self._fb(), stage, self.deform_tables(), c, dt, *g.dx))
+
+    def stage_reinit(self, out, us, uc, stage, dt, phi0=None):
+        g = self.grid
+        self._ok(self._reinit(out.data_ptr(), us.data_ptr(), uc.data_ptr() if uc is not None else None,
+                              *self._gb(), *self._fb(), stage, phi0.data_ptr() if phi0 is not None else None,
+                              1 if phi0 is not None else 0, dt, *g.dx))
+
+    def deform_velocity(self, c):
+        vel = [self.field(None) for _ in range(3)]
+        self._ok(self._velfill(vel[0].data_ptr(), vel[1].data_ptr(), vel[2].data_ptr(), *self._gb(), *self._gb(),
+                               self.deform_tables(), c))
+        return vel
+
+    # -- whole RK3 steps --------------------------------------------------------------
+    def rk3_fused(self, phi, work, stage_fn, dt, **kw):
+        """phi <- TVD-RK3 step; work = (phi1, phi2) scratch fields.  stage_fn(out, us, uc, stage, dt, **kw)."""
+        p1, p2 = work
+        stage_fn(p1, phi, None, 1, dt, **kw)
+        self.fill_ghosts(p1)
+        stage_fn(p2, p1, phi, 2, dt, **kw)
+        self.fill_ghosts(p2)
+        stage_fn(phi, p2, phi, 3, dt, **kw)
+        self.fill_ghosts(phi)
+
+    def volume_phi_lt0(self, phi, eps):
+        out = ctypes.c_double(0.0)
+        d = [ctypes.c_double(v) for v in self.grid.dx] + [ctypes.c_double(eps)]
+        self._ok(self._vol(ctypes.byref(out), phi.data_ptr(), *self._gb(), *self._fb(), *[ctypes.byref(v) for v in d]))
+        return out.value
+
+
+class DropInChain:
+    """The reference's call sequence for one TVD-RK3 step, issued through the
+    drop-in entry points (one full-grid pass per derivative, term and stage)."""
+
+    def __init__(self, solver: Solver):
+        self.s = solver
+        g = solver.grid
+        f = solver.field
+        self.grads = [f() for _ in range(6)]
+        self.rhs = f()
+        self.d1 = f()
+        self.api = solver.lib.api
+
+    def rhs_normal(self, phi, vn):
+        g, a, G = self.s.grid, self.api, self.grads
+        gb, fb = g.gb, g.fb
+        a.lsm3dhjweno5(G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, phi, gb, self.d1, gb, fb, *g.dx)
+        a.lsm3dzerooutlevelseteqnrhs(self.rhs, gb)
+        a.lsm3daddconstnormalveltermtolserhs(self.rhs, gb, G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, vn, fb)
+
+    def rhs_advect(self, phi, vel):
+        g, a, G = self.s.grid, self.api, self.grads
+        gb, fb = g.gb, g.fb
+        a.lsm3dupwindhjweno5(G[0], G[1], G[2], gb, phi, gb, vel[0], vel[1], vel[2], gb, self.d1, gb, fb, *g.dx)
+        a.lsm3dzerooutlevelseteqnrhs(self.rhs, gb)
+        a.lsm3daddadvectiontermtolserhs(self.rhs, gb, G[0], G[1], G[2], gb, vel[0], vel[1], vel[2], gb, fb)
+
+    def rhs_reinit(self, phi, phi0=None):
+        g, a, G = self.s.grid, self.api, self.grads
+        gb, fb = g.gb, g.fb
+        a.lsm3dhjweno5(G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, phi, gb, self.d1, gb, fb, *g.dx)
+        a.lsm3dcomputereinitializationeqnrhs(self.rhs, gb, phi, gb, phi0 if phi0 is not None else phi, gb,
+                                             G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, fb, *g.dx,
+                                             1 if phi0 is not None else 0)
+
+    def rk3(self, phi, work, rhs_fn, dt, **kw):
+        g, a = self.s.grid, self.api
+        gb, fb = g.gb, g.fb
+        p1, p2 = work
+        rhs_fn(phi, **kw)
+        a.lsm3dtvdrk3stage1(p1, gb, phi, gb, self.rhs, gb, fb, dt)
+        self.s.fill_ghosts(p1)
+        rhs_fn(p1, **kw)
+        a.lsm3dtvdrk3stage2(p2, gb, p1, gb, phi, gb, self.rhs, gb, fb, dt)
+        self.s.fill_ghosts(p2)
+        rhs_fn(p2, **kw)
+        a.lsm3dtvdrk3stage3(phi, gb, p2, gb, phi, gb, self.rhs, gb, fb, dt)
+        self.s.fill_ghosts(phi)
+
+
+_CFL_CACHE = {}
+
+
+def advection_cfl_dt(grid: Grid, cfl: float, c: float) -> float:
+    """CFL step of lsm3dComputeStableAdvectionDt for the deformation field
+    scaled by c.  max |u|/dx + |v|/dy + |w|/dz over the interior is |c| times
+    the maximum of the unscaled field, which is evaluated once per grid on the
+    host in FP64 (it is separable per axis)."""
+    key = (grid.n, grid.lo, grid.hi)
+    if key not in _CFL_CACHE:
+        A = [np.sin(np.pi * grid.axis(d)[grid.g:grid.g + grid.n[d]]) ** 2 for d in range(3)]
+        B = [np.abs(np.sin(2 * np.pi * grid.axis(d)[grid.g:grid.g + grid.n[d]])) for d in range(3)]
+        ix, iy, iz = (1.0 / v for v in grid.dx)
+        best = 0.0
+        for k in range(grid.n[2]):
+            s = (2 * A[0][None, :] * B[1][:, None] * B[2][k] * ix + B[0][None, :] * A[1][:, None] * B[2][k] * iy
+                 + B[0][None, :] * B[1][:, None] * A[2][k] * iz)
+            best = max(best, float(s.max()))
+        _CFL_CACHE[key] = best
+    return cfl / (_CFL_CACHE[key] * abs(c) + 1e-36)
+
+
+def deformation_time_factor(t: float, period: float = 3.0) -> float:
+    return math.cos(math.pi * t / period)
