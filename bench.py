@@ -1,0 +1,229 @@
+"""Benchmark of the fused HJ-WENO5 + TVD-RK3 level-set step on one MI355X.
+
+Workload: the 512^3 deformation-advection configuration -- a sphere of radius
+0.15 at (0.35, 0.35, 0.35) on [0,1]^3 advected with HJ-WENO5 + TVD-RK3 at CFL
+0.5 through the divergence-free deformation field times cos(pi t / 3),
+reinitialized every 20 steps (10 HJ-WENO5 + TVD-RK3 pseudo-steps of 0.5 dx).
+A "step" is one advection step; the reinitialization blocks due within the
+timed steps are inside the timed region.  `value` counts every full-grid
+WENO5-RK3 update (advection and reinitialization) of the 512^3 interior.
+
+The headline runs FP32 storage and arithmetic through the fused stage kernels.
+Beside it the same workload is timed through the FP64 fused kernels and the
+FP64 drop-in chain (the reference's call sequence), and the FP32 result is
+compared with the FP64 one (L-inf / L1 of phi and relative interface volume,
+FP64 Kahan-compensated).  Inputs are resident in HBM when timing starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-compare]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "WENO5-RK3 cell-updates/s @512³ + % HBM peak; FP32 vs FP64 at stated L∞ err"
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def run_timed(run, steps):
+    _sync()
+    t0 = time.perf_counter()
+    updates = 0
+    for _ in range(steps):
+        updates += run.step()
+    _sync()
+    return time.perf_counter() - t0, updates
+
+
+def roofline_from_events(run, bytes_per_scalar):
+    """Average duration of the fused advection stage launches (HIP events on
+    the launch stream) and their algorithmic bytes: stage 1 reads phi_n and
+    writes phi_1 (2 passes), stages 2 and 3 read the stage field and phi_n and
+    write the next one (3 passes)."""
+    passes = {"advect1": 2, "advect2": 3, "advect3": 3}
+    cells = run.grid.cells
+    tot_ms, tot_bytes, n = 0.0, 0.0, 0
+    for kind, e0, e1 in run.events:
+        if kind in passes:
+            tot_ms += e0.elapsed_time(e1)
+            tot_bytes += passes[kind] * bytes_per_scalar * cells
+            n += 1
+    avg_ms = tot_ms / max(n, 1)
+    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(), "kernel": "fused_stage_kernel<advect>",
+            "avg_launch_ms": round(avg_ms, 4), "launches": n,
+            "alg_bytes_per_cell_update": 8 * bytes_per_scalar}
+
+
+def _traffic():
+    """HBM bytes per launch of the FP32 advection stage kernel from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "fused_advect_f32_512_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(n=80, steps=2):
+    """The reference's own FP64 kernels (oracle/_ref, flang-built) -- or the
+    oracle restatement when that build is absent -- driven through the same
+    call sequence on the host, one thread, on an n^3 sample of the workload."""
+    from lsmlib_amd import capi
+    from lsmlib_amd.solver import Grid, deformation_tables, deformation_time_factor, phi_sphere
+
+    ref = os.path.join(ROOT, "oracle", "_ref", "liblsm_ref.so")
+    if os.path.exists(ref):
+        lib, kind = capi.Bound(ctypes.CDLL(ref)), "reference"
+    else:
+        lib = capi.Bound(ctypes.CDLL(os.path.join(ROOT, "oracle", "liblsm_oracle.so")), prefix="oracle_",
+                         strip_underscore=True)
+        kind = "port"
+    g = Grid((n, n, n))
+    gb, fb = g.gb, g.fb
+    phi = phi_sphere(g, (0.35, 0.35, 0.35), 0.15)
+    A0, A1, A2, B0, B1, B2 = deformation_tables(g)
+    ux = 2 * A0[None, None, :] * B1[None, :, None] * B2[:, None, None]
+    uy = -(B0[None, None, :] * A1[None, :, None] * B2[:, None, None])
+    uz = -(B0[None, None, :] * B1[None, :, None] * A2[:, None, None])
+    grads = [np.zeros(g.shape) for _ in range(3)]
+    d1, rhs, p1, p2 = (np.zeros(g.shape) for _ in range(4))
+    from lsmlib_amd.solver import advection_cfl_dt
+    dt = advection_cfl_dt(g, 0.5, 1.0)
+
+    def fill(a):
+        for idx in range(6):
+            lib.lsm3dsignedlinearextrapolation(a, gb, fb, idx)
+
+    fill(phi)
+    t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for st, (src, out, tt) in enumerate(((phi, p1, t), (p1, p2, t + dt), (p2, phi, t + 0.5 * dt)), start=1):
+            c = deformation_time_factor(tt)
+            vel = (ux * c, uy * c, uz * c)
+            lib.lsm3dupwindhjweno5(*grads, gb, src, gb, *vel, gb, d1, gb, fb, *g.dx)
+            lib.lsm3dzerooutlevelseteqnrhs(rhs, gb)
+            lib.lsm3daddadvectiontermtolserhs(rhs, gb, *grads, gb, *vel, gb, fb)
+            if st == 1:
+                lib.lsm3dtvdrk3stage1(out, gb, phi, gb, rhs, gb, fb, dt)
+            elif st == 2:
+                lib.lsm3dtvdrk3stage2(out, gb, src, gb, phi, gb, rhs, gb, fb, dt)
+            else:
+                lib.lsm3dtvdrk3stage3(out, gb, src, gb, phi, gb, rhs, gb, fb, dt)
+            fill(out)
+        t += dt
+    el = time.perf_counter() - t0
+    return {"value": round(g.cells * steps / el, 1), "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": "deformation advection, %d^3, %d HJ-WENO5 + TVD-RK3 steps (FP64, serial), %.1f s" % (
+                n, steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 timings and the accuracy check")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=64)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box of this build has one MI355X and the path is not sharded "
+              "across GPUs (the slab decomposition is described in DESIGN.md); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from lsmlib_amd.solver import DeformationRun
+
+    print("[bench] building the %d^3 deformation workload (FP32 fused)" % a.n, file=sys.stderr, flush=True)
+    run32 = DeformationRun(a.n, "f32")
+    for _ in range(a.warmup):
+        run32.step()
+    # warm-up steps advance time; restart the clock of the field from the same state
+    run32.events = []
+    el, updates = run_timed(run32, a.steps)
+    run32.solver.lib.check()
+    value = run32.grid.cells * updates / el
+    roof = roofline_from_events(run32, 4)
+    print("[bench] FP32 fused: %.3f ms/step, %.3e cell-updates/s" % (1e3 * el / a.steps, value), file=sys.stderr,
+          flush=True)
+
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "deformation advection of a sphere r=0.15 at (0.35,0.35,0.35), %d^3 on [0,1]^3, "
+                                  "HJ-WENO5 + TVD-RK3 at CFL 0.5, reinitialization (10 WENO5-RK3 pseudo-steps of "
+                                  "0.5dx) every 20 steps, signed linear extrapolation ghosts" % a.n,
+                      "grid": a.n, "full_grid_updates_timed": updates, "parallelism": "single GPU"},
+           "roofline": roof}
+
+    if not a.no_compare:
+        torch.cuda.empty_cache()
+        run64 = DeformationRun(a.n, "f64")
+        for _ in range(a.warmup):
+            run64.step()
+        el64, up64 = run_timed(run64, a.steps)
+        run64.solver.lib.check()
+        v64 = run64.grid.cells * up64 / el64
+        print("[bench] FP64 fused: %.3f ms/step" % (1e3 * el64 / a.steps), file=sys.stderr, flush=True)
+        # accuracy: FP32 against FP64 after the same warmup + timed steps from the same initial field
+        p32 = run32.grid.interior(run32.solver.host(run32.phi))
+        p64 = run64.grid.interior(run64.solver.host(run64.phi))
+        eps = 1.5 * run64.grid.dx[0]
+        v_32 = run32.solver.volume_phi_lt0(run32.phi, eps)
+        v_64 = run64.solver.volume_phi_lt0(run64.phi, eps)
+        band = np.abs(p64) < 6 * run64.grid.dx[0]
+        acc = {"steps": a.warmup + a.steps, "linf_phi": float(np.abs(p32 - p64).max()),
+               "l1_phi": float(np.abs(p32 - p64).mean()),
+               "linf_phi_band6dx": float(np.abs(p32 - p64)[band].max()) if band.any() else 0.0,
+               "volume_rel_err": abs(v_32 - v_64) / abs(v_64), "volume_f64": v_64, "volume_f32": v_32}
+        del p32, p64, band, run64
+        torch.cuda.empty_cache()
+        chain = DeformationRun(a.n, "f64", dropin=True)
+        ksteps = max(1, min(a.steps, 20))
+        for _ in range(1):
+            chain.step()
+        elc, upc = run_timed(chain, ksteps)
+        chain.solver.lib.check()
+        vchain = chain.grid.cells * upc / elc
+        print("[bench] FP64 drop-in chain: %.3f ms/step" % (1e3 * elc / ksteps), file=sys.stderr, flush=True)
+        del chain
+        torch.cuda.empty_cache()
+        out["fp64_fused"] = {"value": round(v64, 1), "ms_per_step": round(1e3 * el64 / a.steps, 4)}
+        out["fp64_dropin_chain"] = {"value": round(vchain, 1), "ms_per_step": round(1e3 * elc / ksteps, 4),
+                                    "steps": ksteps}
+        out["speedup_fp32_fused_vs_fp64_dropin"] = round(value / vchain, 2)
+        out["speedup_fp32_fused_vs_fp64_fused"] = round(value / v64, 2)
+        out["accuracy_fp32_vs_fp64"] = acc
+
+    if not a.no_cpu:
+        print("[bench] CPU baseline (reference kernels, 1 thread)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -312,3 +312,90 @@ def advection_cfl_dt(grid: Grid, cfl: float, c: float) -> float:
 
 def deformation_time_factor(t: float, period: float = 3.0) -> float:
     return math.cos(math.pi * t / period)
+
+
+# ---------------------------------------------------------------------------
+# Deformation-advection workload (sphere through the time-reversing field)
+# ---------------------------------------------------------------------------
+class DeformationRun:
+    """A sphere of radius 0.15 at (0.35, 0.35, 0.35) on [0,1]^3 advected by the
+    divergence-free deformation field times cos(pi t / 3) with HJ-WENO5 +
+    TVD-RK3; the field is evaluated at the stage times t, t+dt, t+dt/2.  dt is
+    the CFL-0.5 advection step of the unscaled field.  Every `reinit_every`
+    steps phi is reinitialized with HJ-WENO5 + TVD-RK3 pseudo-steps of
+    0.5*dx up to 5*dx (the schedule of
+    examples/curvature_driven_motion/curvature_model3d.c:484-624)."""
+
+    def __init__(self, n=512, precision="f32", bc="signed_linear", reinit_every=20, cfl=0.5, dropin=False):
+        self.grid = Grid((n, n, n))
+        self.solver = Solver(self.grid, precision, bc=bc)
+        self.reinit_every = reinit_every
+        self.dt = advection_cfl_dt(self.grid, cfl, 1.0)
+        self.dt_r = 0.5 * min(self.grid.dx)
+        self.reinit_steps = int(round(5.0 / 0.5))
+        s = self.solver
+        self.phi = s.field(phi_sphere(self.grid, (0.35, 0.35, 0.35), 0.15))
+        s.fill_ghosts(self.phi)
+        self.work = (s.field(), s.field())
+        self.t = 0.0
+        self.nsteps = 0
+        self.chain = DropInChain(s) if dropin else None
+        self.events = None  # list of (stage kind, start event, end event) when instrumented
+
+    def _stage(self, kind, fn, *a, **kw):
+        if self.events is None:
+            return fn(*a, **kw)
+        t = self.solver.torch
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record()
+        fn(*a, **kw)
+        e1.record()
+        self.events.append((kind, e0, e1))
+
+    def advect_step(self):
+        s, phi, (p1, p2), dt, t = self.solver, self.phi, self.work, self.dt, self.t
+        cs = [deformation_time_factor(t), deformation_time_factor(t + dt), deformation_time_factor(t + 0.5 * dt)]
+        if self.chain is None:
+            self._stage("advect1", s.stage_deform, p1, phi, None, 1, dt, cs[0])
+            s.fill_ghosts(p1)
+            self._stage("advect2", s.stage_deform, p2, p1, phi, 2, dt, cs[1])
+            s.fill_ghosts(p2)
+            self._stage("advect3", s.stage_deform, phi, p2, phi, 3, dt, cs[2])
+            s.fill_ghosts(phi)
+        else:
+            g, a, ch = self.grid, s.lib.api, self.chain
+            gb, fb = g.gb, g.fb
+            for st, (src, out, c) in enumerate(((phi, p1, cs[0]), (p1, p2, cs[1]), (p2, phi, cs[2])), start=1):
+                vel = s.deform_velocity(c)
+                ch.rhs_advect(src, vel=vel)
+                if st == 1:
+                    a.lsm3dtvdrk3stage1(out, gb, phi, gb, ch.rhs, gb, fb, dt)
+                elif st == 2:
+                    a.lsm3dtvdrk3stage2(out, gb, src, gb, phi, gb, ch.rhs, gb, fb, dt)
+                else:
+                    a.lsm3dtvdrk3stage3(out, gb, src, gb, phi, gb, ch.rhs, gb, fb, dt)
+                s.fill_ghosts(out)
+        self.t += dt
+        self.nsteps += 1
+
+    def reinitialize(self):
+        s, (p1, p2) = self.solver, self.work
+        for _ in range(self.reinit_steps):
+            if self.chain is None:
+                self._stage("reinit1", s.stage_reinit, p1, self.phi, None, 1, self.dt_r)
+                s.fill_ghosts(p1)
+                self._stage("reinit2", s.stage_reinit, p2, p1, self.phi, 2, self.dt_r)
+                s.fill_ghosts(p2)
+                self._stage("reinit3", s.stage_reinit, self.phi, p2, self.phi, 3, self.dt_r)
+                s.fill_ghosts(self.phi)
+            else:
+                self.chain.rk3(self.phi, self.work, self.chain.rhs_reinit, self.dt_r)
+
+    def step(self):
+        """One advection step, plus the reinitialization block when due.
+        Returns the number of full-grid WENO5-RK3 updates performed."""
+        self.advect_step()
+        if self.reinit_every and self.nsteps % self.reinit_every == 0:
+            self.reinitialize()
+            return 1 + self.reinit_steps
+        return 1
