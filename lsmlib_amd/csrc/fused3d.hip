@@ -56,6 +56,21 @@ __device__ __forceinline__ void deform_velocity(const real_t* const ta[3], const
   w = -(tb[0][ii] * tb[1][jj] * ta[2][kk] * c);
 }
 
+// WENO5 derivatives of the fused stage: the reference's operation order in
+// FP64 (bitwise equal to the drop-in chain), the rearranged FP32 forms in FP32.
+__device__ __forceinline__ void weno_pm(const double f[7], double inv, double& p, double& m) {
+  stencil::plus_minus<5>(f, inv, p, m);
+}
+__device__ __forceinline__ void weno_pm(const float f[7], float inv, float& p, float& m) {
+  stencil::fastf::plus_minus(f, inv, p, m);
+}
+__device__ __forceinline__ double weno_up(const double f[7], double inv, bool vpos) {
+  return stencil::upwind<5>(f, inv, vpos);
+}
+__device__ __forceinline__ float weno_up(const float f[7], float inv, bool vpos) {
+  return stencil::fastf::upwind(f, inv, vpos);
+}
+
 template <int STAGE>
 __device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, real_t r) {
   if (STAGE == 1) return uc + dt * r;
@@ -84,13 +99,26 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
   const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
 
   const Box gb = a.gb;
+  // Staging slots of this thread in a plane tile (the same for every plane):
+  // LDS element e = tid + r*TX*TY and its in-plane global offset, or -1 when
+  // the element lies outside the ghost box (never read by an active cell).
+  constexpr int NSLOT = (PLANE + TX * TY - 1) / (TX * TY);
+  int soff[NSLOT];
+#pragma unroll
+  for (int r = 0; r < NSLOT; r++) {
+    const int e = tid + r * TX * TY;
+    const int gi = i0 - H + e % PX, gj = j0 - H + e / PX;
+    soff[r] = (e < PLANE && gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
+                  ? (gi - gb.ilo) + a.sy * (gj - gb.jlo)
+                  : -1;
+  }
   auto load_plane = [&](int kp) {
+    if (kp < gb.klo || kp > gb.khi) return;  // uniform across the block
     real_t* dst = ring + ((kp - kb + 3 * RING) % RING) * PLANE;
-    for (int e = tid; e < PLANE; e += TX * TY) {
-      const int gi = i0 - H + e % PX, gj = j0 - H + e / PX;
-      if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi && kp >= gb.klo && kp <= gb.khi)
-        dst[e] = a.us[(gi - gb.ilo) + a.sy * (gj - gb.jlo) + a.sz * (kp - gb.klo)];
-    }
+    const real_t* src = a.us + a.sz * (kp - gb.klo);
+#pragma unroll
+    for (int r = 0; r < NSLOT; r++)
+      if (soff[r] >= 0) dst[tid + r * TX * TY] = src[soff[r]];
   };
   for (int kp = kb - H; kp < kb + H; kp++) load_plane(kp);
 
@@ -118,15 +146,15 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
         } else {
           deform_velocity(a.ta, a.tb, i - gb.ilo, j - gb.jlo, k - gb.klo, a.tfac, u, v, w);
         }
-        const real_t px = fabs(u) < kZeroTol ? real_t(0) : stencil::upwind<5>(fx, a.ix, u > real_t(0));
-        const real_t py = fabs(v) < kZeroTol ? real_t(0) : stencil::upwind<5>(fy, a.iy, v > real_t(0));
-        const real_t pz = fabs(w) < kZeroTol ? real_t(0) : stencil::upwind<5>(fz, a.iz, w > real_t(0));
+        const real_t px = fabs(u) < kZeroTol ? real_t(0) : weno_up(fx, a.ix, u > real_t(0));
+        const real_t py = fabs(v) < kZeroTol ? real_t(0) : weno_up(fy, a.iy, v > real_t(0));
+        const real_t pz = fabs(w) < kZeroTol ? real_t(0) : weno_up(fz, a.iz, w > real_t(0));
         r = real_t(0) - (u * px + v * py + w * pz);
       } else {
         real_t xp, xm, yp, ym, zp, zm;
-        stencil::plus_minus<5>(fx, a.ix, xp, xm);
-        stencil::plus_minus<5>(fy, a.iy, yp, ym);
-        stencil::plus_minus<5>(fz, a.iz, zp, zm);
+        weno_pm(fx, a.ix, xp, xm);
+        weno_pm(fy, a.iy, yp, ym);
+        weno_pm(fz, a.iz, zp, zm);
         if (RHS == kReinit) {
           const real_t s = a.use_phi0 ? a.phi0[off] : fx[3];
           r = stencil::reinit_rhs(s, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);

@@ -85,6 +85,7 @@ enum ErrorCode {
 
 void set_error(int code, const char* fmt, ...);
 hipStream_t stream();
+bool synchronous();
 // Ends an entry point: records a launch error and, in synchronous mode (the
 // default, matching the reference's completion semantics), waits for the stream.
 void finish(const char* who);

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(kThreads) sum_kernel(Box b, Op op, double* par
 }
 
 // One block: combines nblocks partial pairs in a fixed order.
-__global__ void __launch_bounds__(kThreads) sum_finish_kernel(const double* partials, int nblocks, double* out) {
+static __global__ void __launch_bounds__(kThreads) sum_finish_kernel(const double* partials, int nblocks, double* out) {
   Pair p{0.0, 0.0};
   for (int t = threadIdx.x; t < nblocks; t += kThreads) p = add(p, Pair{partials[2 * t], partials[2 * t + 1]});
   p = wave_sum(p);

@@ -38,6 +38,7 @@ void set_error(int code, const char* fmt, ...) {
 }
 
 hipStream_t stream() { return g_stream; }
+bool synchronous() { return g_sync; }
 
 bool check_hip(hipError_t e, const char* who) {
   if (e == hipSuccess) return true;

@@ -218,5 +218,66 @@ __device__ __forceinline__ T normal_h_over_dx(T absv, T xp, T yp, T zp, T xm, T 
   return absv / n * (x * ix + y * iy + z * iz);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 fused-path forms: the same HJ-WENO5 scheme rearranged for fewer VALU
+// instructions (FP32 results are judged by their error against FP64, not by
+// bitwise equality).
+//  * The reconstruction is evaluated on undivided differences and scaled by
+//    1/dx once: the weights are invariant under a common scaling of v.
+//  * With m = max(v_k^2) the weights alpha_k = d_k / (S_k + 1e-6 m)^2 equal
+//    d_k / (S_k/m + 1e-6)^2 up to a common factor; S_k/m lies in [0, 34], so
+//    the normalised terms stay far from FP32 overflow and underflow.  (The
+//    reference's additive 1e-36 only matters when every |v| < 1e-15, where all
+//    candidate stencils vanish.)
+//  * The three weights are put over a common denominator, so the reconstruction
+//    needs two hardware reciprocals (v_rcp_f32, 1 ulp) instead of four IEEE
+//    divisions.
+// ---------------------------------------------------------------------------
+namespace fastf {
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
+  // max(v_k^2) == (max |v_k|)^2 exactly (rounding is monotonic): two max3 and a multiply
+  const float ma = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fabsf(v3)), fmaxf(fabsf(v4), fabsf(v5)));
+  const float im = rcp(fmaxf(ma * ma, 1e-30f));
+  const float a1 = fmaf(-2.f, v2, v1) + v3, b1 = fmaf(3.f, v3, fmaf(-4.f, v2, v1));
+  const float a2 = fmaf(-2.f, v3, v2) + v4, b2 = v2 - v4;
+  const float a3 = fmaf(-2.f, v4, v3) + v5, b3 = fmaf(3.f, v3, fmaf(-4.f, v4, v5));
+  const float c13 = 13.f / 12.f;
+  const float s1 = fmaf(c13, a1 * a1, 0.25f * (b1 * b1));
+  const float s2 = fmaf(c13, a2 * a2, 0.25f * (b2 * b2));
+  const float s3 = fmaf(c13, a3 * a3, 0.25f * (b3 * b3));
+  const float e1 = fmaf(s1, im, 1e-6f), e2 = fmaf(s2, im, 1e-6f), e3 = fmaf(s3, im, 1e-6f);
+  const float q1 = e1 * e1, q2 = e2 * e2, q3 = e3 * e3;
+  const float w1 = 0.1f * (q2 * q3), w2 = 0.6f * (q1 * q3), w3 = 0.3f * (q1 * q2);
+  const float p1 = fmaf(1.f / 3.f, v1, fmaf(-7.f / 6.f, v2, (11.f / 6.f) * v3));
+  const float p2 = fmaf(-1.f / 6.f, v2, fmaf(5.f / 6.f, v3, (1.f / 3.f) * v4));
+  const float p3 = fmaf(1.f / 3.f, v3, fmaf(5.f / 6.f, v4, (-1.f / 6.f) * v5));
+  const float num = fmaf(w1, p1, fmaf(w2, p2, w3 * p3));
+  return num * rcp(w1 + w2 + w3);
+}
+
+// Plus and minus WENO5 derivatives from seven samples f[s+3] along an axis.
+__device__ __forceinline__ void plus_minus(const float f[7], float inv, float& p, float& m) {
+  float d[6];
+#pragma unroll
+  for (int s = -2; s <= 3; s++) d[s + 2] = f[s + 3] - f[s + 2];
+  p = weno5(d[5], d[4], d[3], d[2], d[1]) * inv;
+  m = weno5(d[0], d[1], d[2], d[3], d[4]) * inv;
+}
+
+// Velocity-upwinded WENO5 derivative (vpos: backward/minus stencil).
+__device__ __forceinline__ float upwind(const float f[7], float inv, bool vpos) {
+  float d[6];
+#pragma unroll
+  for (int s = -2; s <= 3; s++) d[s + 2] = f[s + 3] - f[s + 2];
+  const float a = vpos ? d[0] : d[5], b = vpos ? d[1] : d[4], c = vpos ? d[2] : d[3];
+  const float e = vpos ? d[3] : d[2], g = vpos ? d[4] : d[1];
+  return weno5(a, b, c, e, g) * inv;
+}
+
+}  // namespace fastf
+
 }  // namespace stencil
 }  // namespace lsmhip

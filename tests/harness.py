@@ -194,7 +194,7 @@ def sphere_phi(gb, centre, radius, h):
 
 
 def band_case(impl=None, seed=0, n=(24, 22, 20), level=4, h=None, width=None, width_inner=None,
-              marks=(124, 125, 126, 127)):
+              marks=(124, 125, 126, 127), device=None):
     """A narrow band around two spheres, one of them cut by the domain
     boundary.  The band (levels 0..level) is built with ``impl`` (the oracle
     by default) and the boundary layers are marked as the reference's local
@@ -220,7 +220,7 @@ def band_case(impl=None, seed=0, n=(24, 22, 20), level=4, h=None, width=None, wi
     c.boxes.update(nb=gb, gb=gb)
     c.scalars.update(nlo=0, nhi=npts - 1, nlo_outer=0, nhi_outer=npts - 1, level=level,
                      width=width or 6 * h, width_inner=width_inner or 3 * h)
-    arrs, outs = run(impl, "lsm3ddeterminenarrowband_", c)
+    arrs, outs = run(impl, "lsm3ddeterminenarrowband_", c, device=device)
     for k in ("nb", "ix", "iy", "iz", "outer", "n_lo", "n_hi"):
         c.arrays[k] = arrs[k]
     (c.scalars["nlo_outer_plus"], c.scalars["nhi_outer_plus"], c.scalars["nlo_outer_minus"],
@@ -230,7 +230,7 @@ def band_case(impl=None, seed=0, n=(24, 22, 20), level=4, h=None, width=None, wi
                       (gb, mark_gb)):
         c.boxes["fb"] = box
         c.scalars["mark"] = mark
-        arrs, _ = run(impl, "lsm3dmarknarrowbandboundarylayer_", c)
+        arrs, _ = run(impl, "lsm3dmarknarrowbandboundarylayer_", c, device=device)
         c.arrays["nb"] = arrs["nb"]
     c.boxes["fb"] = fb
     c.scalars.update(mark_fb=mark_fb, mark_D1=mark_d1, mark_D2=mark_d2)

@@ -37,6 +37,7 @@ def test_determine_narrow_band_from_mask():
         c.arrays["mask"] = c.arrays["phi"] - 0.1
         c.boxes["mask"] = c.gb
         c.scalars["use_mask_sign"] = use_sign
+        c.scalars.update(nlo=0, nhi=c.arrays["ix"].size - 1)
         H.assert_same(H.run(H.reference(), "lsm3ddeterminenarrowbandfrommask_", c),
                       H.run(H.oracle(), "lsm3ddeterminenarrowbandfrommask_", c), "frommask")
 
