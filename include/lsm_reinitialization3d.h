@@ -1,0 +1,25 @@
+/*
+ * lsm_reinitialization3d.h -- reinitialization RHS on the GPU
+ * (reference: src/reinitialization/lsm_reinitialization3d.h).
+ */
+#ifndef LSMHIP_REINITIALIZATION_3D_H
+#define LSMHIP_REINITIALIZATION_3D_H
+
+#include "lsmhip_config.h"
+
+#define LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS lsm3dcomputereinitializationeqnrhs_
+
+LSMHIP_BEGIN_DECLS
+
+/* replaces src/reinitialization/lsm_reinitialization3d.h:77 */
+void LSM3D_COMPUTE_REINITIALIZATION_EQN_RHS(
+    LSMLIB_REAL *reinit_rhs, LSMHIP_BOX_ARGS(rhs_gb), const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(phi_gb),
+    const LSMLIB_REAL *phi0, LSMHIP_BOX_ARGS(phi0_gb), const LSMLIB_REAL *phi_x_plus, const LSMLIB_REAL *phi_y_plus,
+    const LSMLIB_REAL *phi_z_plus, LSMHIP_BOX_ARGS(grad_phi_plus_gb), const LSMLIB_REAL *phi_x_minus,
+    const LSMLIB_REAL *phi_y_minus, const LSMLIB_REAL *phi_z_minus, LSMHIP_BOX_ARGS(grad_phi_minus_gb),
+    LSMHIP_BOX_ARGS(fb), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy, const LSMLIB_REAL *dz,
+    const int *use_phi0_for_sgn);
+
+LSMHIP_END_DECLS
+
+#endif

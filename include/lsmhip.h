@@ -1,0 +1,101 @@
+/*
+ * lsmhip.h -- entry points of the HIP level-set library that have no
+ * counterpart in the reference: library control, the fused WENO5 + RHS +
+ * TVD-RK stage kernels, ghost-fill dispatch (including the periodic fill the
+ * serial reference lacks) and FP64-accumulated diagnostics.
+ *
+ * Include this with the reference-named headers (lsm_*3d*.h) of the drop-in.
+ */
+#ifndef LSMHIP_H
+#define LSMHIP_H
+
+#include "lsmhip_config.h"
+
+LSMHIP_BEGIN_DECLS
+
+/* ---- error side channel (the reference's kernels return nothing and check nothing) ---- */
+#define LSMHIP_OK 0
+#define LSMHIP_ERR_BOX 1   /* a fill/include box is not inside an array's ghost box (nothing is done) */
+#define LSMHIP_ERR_HIP 2   /* a HIP runtime call failed */
+#define LSMHIP_ERR_ARG 3   /* invalid argument, or index lists too short for a band */
+#define LSMHIP_ERR_INDEX 4 /* index-list points whose stencil left a ghost box were skipped */
+#define LSMHIP_ERR_SIZE 5  /* an array exceeds 2^31 elements */
+int lsmhip_get_last_error(void);
+const char *lsmhip_get_last_error_string(void);
+void lsmhip_clear_error(void);
+/* Reads (and clears) the device flag of index-list kernels; needed after
+ * calls made in stream-ordered mode.  Returns nonzero when points were skipped. */
+int lsmhip_check_device_errors(void);
+
+/* ---- execution ---- */
+/* stream: a hipStream_t (NULL = the null stream).  synchronous != 0 (the
+ * default) waits for every call to complete, as the reference's calls do;
+ * 0 leaves calls stream-ordered (results needed on the host, such as CFL
+ * steps and integrals, still synchronise). */
+void lsmhip_set_stream(void *stream, int synchronous);
+void *lsmhip_get_stream(void);
+int lsmhip_synchronize(void);
+/* sizeof(LSMLIB_REAL) of the loaded library: 8 (liblsmhip.so) or 4 (liblsmhip_f32.so). */
+int lsmhip_real_size(void);
+
+/* ---- ghost cells ---- */
+/* Applies one fill to the face group of the reference's C helpers
+ * (src/boundary_conditions/lsm_boundary_conditions.h:60-80): 0..5 one face,
+ * 6 x faces, 7 y faces, 8 z faces, 9 all faces in the order x_lo, x_hi,
+ * y_lo, y_hi, z_lo, z_hi.  kind: 0 linear, 1 signed linear, 2 copy
+ * (= homogeneous Neumann), 3 periodic.  Returns 0 or -1. */
+int lsmhip_fill_ghosts(LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), int bdry_location_idx,
+                       int kind);
+/* Periodic fill of one face (index 0..5): ghost layers copy the interior
+ * layers one period away; faces applied x, y, z fill edges and corners too. */
+void lsmhip_periodic_face_(LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), const int *bdry_location_idx);
+
+/* ---- fused stages: one TVD-RK stage of phi_t + H = 0 in one kernel ----
+ * All arrays are device arrays on the ghost box gb (phi_stage with its ghost
+ * cells filled); the stage writes phi_out on fb only.  stage 1, 2, 3 are the
+ * TVD-RK3 stages (u_next = u + dt L(u);  3/4 u_n + 1/4 (u_1 + dt L(u_1));
+ * 1/3 u_n + 2/3 (u_2 + dt L(u_2))), 22 is TVD-RK2 stage 2.  phi_n (the step's
+ * initial field) may be NULL for stage 1.  FP64: bitwise equal to the drop-in
+ * chain named in each comment.  FP32: the same scheme in FP32 storage and
+ * arithmetic, arranged for fewer instructions.  Returns 0 or -1. */
+
+/* HJ_WENO5 -> ZERO_OUT + ADD_[CONST_]NORMAL_VEL_TERM -> RK stage; vn_field NULL: constant vn. */
+int lsmhip_weno5_stage_normal(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                              LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), int stage, LSMLIB_REAL vn,
+                              const LSMLIB_REAL *vn_field, LSMLIB_REAL dt, LSMLIB_REAL dx, LSMLIB_REAL dy,
+                              LSMLIB_REAL dz);
+/* UPWIND_HJ_WENO5 -> ZERO_OUT + ADD_ADVECTION_TERM -> RK stage; velocity arrays on gb. */
+int lsmhip_weno5_stage_advect(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                              LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), int stage, const LSMLIB_REAL *vel_x,
+                              const LSMLIB_REAL *vel_y, const LSMLIB_REAL *vel_z, LSMLIB_REAL dt, LSMLIB_REAL dx,
+                              LSMLIB_REAL dy, LSMLIB_REAL dz);
+/* As above with the divergence-free deformation velocity
+ *   u = 2 A(x) B(y) B(z) c, v = -B(x) A(y) B(z) c, w = -B(x) B(y) A(z) c,
+ *   A = sin^2(pi s), B = sin(2 pi s),
+ * evaluated in the kernel from per-axis device tables indexed from the ghost
+ * box origin: tabs = {A_x, A_y, A_z, B_x, B_y, B_z} (host array of 6 device
+ * pointers); c is the time factor.  No velocity array is read. */
+int lsmhip_weno5_stage_advect_deform(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                                     LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), int stage,
+                                     const LSMLIB_REAL *const *tabs, LSMLIB_REAL c, LSMLIB_REAL dt, LSMLIB_REAL dx,
+                                     LSMLIB_REAL dy, LSMLIB_REAL dz);
+/* HJ_WENO5 -> COMPUTE_REINITIALIZATION_EQN_RHS -> RK stage. */
+int lsmhip_weno5_stage_reinit(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                              LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), int stage, const LSMLIB_REAL *phi0,
+                              int use_phi0_for_sgn, LSMLIB_REAL dt, LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz);
+/* Writes the deformation velocity of lsmhip_weno5_stage_advect_deform into
+ * three arrays on gb over box b. */
+int lsmhip_deform_velocity_fill(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_REAL *vel_z, LSMHIP_BOX_ARGS(gb),
+                                LSMHIP_BOX_ARGS(b), const LSMLIB_REAL *const *tabs, LSMLIB_REAL c);
+/* k planes a fused-stage block marches through (default 64). */
+void lsmhip_set_fused_kchunk(int planes);
+
+/* ---- diagnostics ---- */
+/* Smoothed volume of {phi < 0} (the reference's lsm3dVolumeRegionPhiLessThanZero
+ * integrand) accumulated and returned in FP64 whatever LSMLIB_REAL is. */
+int lsmhip_volume_phi_lt0_f64(double *volume, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(phi_gb), LSMHIP_BOX_ARGS(ib),
+                              const double *dx, const double *dy, const double *dz, const double *epsilon);
+
+LSMHIP_END_DECLS
+
+#endif

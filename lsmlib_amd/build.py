@@ -26,7 +26,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-COMMON = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden",
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+COMMON = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-I" + INCLUDE,
           # packed FP32 VALU forms are kept out of every kernel
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
@@ -49,7 +50,7 @@ def _compile(src, obj, flags):
 
 def build(jobs: int = 8, verbose: bool = False) -> list[str]:
     os.makedirs(BUILD, exist_ok=True)
-    headers = glob.glob(os.path.join(CSRC, "*.hpp"))
+    headers = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(INCLUDE, "*.h"))
     newest_header = max([os.path.getmtime(h) for h in headers] + [0])
     tasks = []
     for lib, flags in VARIANTS.items():

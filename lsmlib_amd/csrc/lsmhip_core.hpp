@@ -22,6 +22,11 @@
 
 typedef LSMHIP_REAL real_t;
 
+// The public headers (include/) declare every exported entry point; including
+// them here makes the compiler check each definition against its declaration.
+#define LSMLIB_REAL LSMHIP_REAL
+#include "lsmhip_all.h"
+
 namespace lsmhip {
 
 constexpr bool kDouble = sizeof(real_t) == 8;
