@@ -16,7 +16,7 @@ import numpy as np
 from lsmlib_amd import capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liblsm_oracle.so")
+ORACLE_SO = os.environ.get("LSMHIP_ORACLE_SO", os.path.join(ROOT, "oracle", "liblsm_oracle.so"))
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "liblsm_ref.so")
 
 _cache = {}
@@ -79,11 +79,14 @@ class Case:
         return self.arrays[name]
 
 
-def _impl_copy(a, device):
+def _impl_copy(a, device, real32=False):
     if device is None:
         return np.array(a, copy=True)
     import torch
-    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if real32 and t.dtype == torch.float64:
+        t = t.to(torch.float32)
+    return t.to(device)
 
 
 def _back(a):
@@ -92,8 +95,9 @@ def _back(a):
     return a.cpu().numpy()
 
 
-def run(impl, sym, case, device=None, skip_arrays=()):
-    """Call impl.<sym> on copies of the case's inputs; return (arrays, outputs)."""
+def run(impl, sym, case, device=None, skip_arrays=(), real32=False):
+    """Call impl.<sym> on copies of the case's inputs; return (arrays, outputs).
+    real32: real arrays go to the device as float32 (liblsmhip_f32.so)."""
     spec = capi.SPECS[sym]
     boxes = box_of_array(sym)
     arrs = {}
@@ -101,7 +105,7 @@ def run(impl, sym, case, device=None, skip_arrays=()):
     for kind, name in spec:
         if kind in ("R", "M"):
             if name not in arrs:
-                arrs[name] = _impl_copy(case.array(name, boxes[name], kind), device)
+                arrs[name] = _impl_copy(case.array(name, boxes[name], kind), device, real32 and kind == "R")
             args.append(arrs[name])
         elif kind == "IA":
             if name not in arrs:
