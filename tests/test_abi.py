@@ -4,6 +4,8 @@ import ctypes
 import glob
 import os
 import re
+import shutil
+import subprocess
 
 import pytest
 
@@ -47,3 +49,42 @@ def test_loader_reports_precision():
         lib = lsmlib_amd.load(prec)
         assert lib.cdll.lsmhip_real_size() == size
         assert lib.api.missing == []
+
+
+def _expanded_prototypes():
+    """Prototypes of include/lsmhip_all.h after the C preprocessor: name ->
+    list of parameter types."""
+    out = subprocess.run(["gcc", "-E", "-P", "-I", os.path.join(ROOT, "include"),
+                          os.path.join(ROOT, "include", "lsmhip_all.h")], capture_output=True, text=True, check=True)
+    txt = " ".join(out.stdout.split())
+    protos = {}
+    for name, params in re.findall(r"\b(lsm\w+)\s*\(([^;{}]*)\)\s*;", txt):
+        types = []
+        for prm in params.split(","):
+            prm = prm.strip()
+            if prm and prm != "void":
+                types.append(re.sub(r"\s*\b\w+$", "", prm).replace(" ", ""))
+        protos[name] = types
+    return protos
+
+
+# spec kind -> parameter type in the FP64 headers
+_KIND_TYPE = {"R": ("double*", "constdouble*"), "S": ("constdouble*",), "SO": ("double*",),
+              "I": ("constint*",), "IO": ("int*",), "IH": ("int*", "constint*"), "IA": ("int*", "constint*"),
+              "M": ("unsignedchar*", "constunsignedchar*"), "U8": ("constunsignedchar*",)}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs the C preprocessor")
+def test_prototypes_match_the_ctypes_table():
+    """Every drop-in prototype has, position by position, the parameter kind
+    the ctypes table passes (boxes are six const int pointers)."""
+    protos = _expanded_prototypes()
+    for sym, spec in capi.SPECS.items():
+        assert sym in protos, sym
+        want = []
+        for kind, name in spec:
+            want.extend([("constint*",)] * 6 if kind == "B" else [_KIND_TYPE[kind]])
+        got = protos[sym]
+        assert len(got) == len(want), (sym, len(got), len(want))
+        for pos, (g, w) in enumerate(zip(got, want)):
+            assert g in w, (sym, pos, g, w)
