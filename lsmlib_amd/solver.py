@@ -42,11 +42,15 @@ class SplitMix64:
 
 
 class Grid:
-    def __init__(self, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), ghost=3):
+    """n interior points per axis on [lo, hi], `ghost` layers.  Node i sits at
+    lo + i*dx (the reference's Grid); centred=True puts it at lo + (i+1/2)*dx."""
+
+    def __init__(self, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), ghost=3, centred=False):
         self.n = tuple(int(v) for v in n)
         self.lo = tuple(float(v) for v in lo)
         self.hi = tuple(float(v) for v in hi)
         self.g = int(ghost)
+        self.centred = bool(centred)
         self.dx = tuple((self.hi[d] - self.lo[d]) / self.n[d] for d in range(3))
         g = self.g
         self.gb = (-g, self.n[0] - 1 + g, -g, self.n[1] - 1 + g, -g, self.n[2] - 1 + g)
@@ -59,7 +63,8 @@ class Grid:
 
     def axis(self, d):
         """Coordinates of the ghost-box index range along axis d (float64)."""
-        return self.lo[d] + self.dx[d] * np.arange(-self.g, self.n[d] + self.g, dtype=np.float64)
+        i = np.arange(-self.g, self.n[d] + self.g, dtype=np.float64)
+        return self.lo[d] + self.dx[d] * (i + 0.5 if self.centred else i)
 
     def mesh(self):
         z, y, x = np.meshgrid(self.axis(2), self.axis(1), self.axis(0), indexing="ij")
@@ -399,3 +404,39 @@ class DeformationRun:
             self.reinitialize()
             return 1 + self.reinit_steps
         return 1
+
+
+# ---------------------------------------------------------------------------
+# Reinitialization workload
+# ---------------------------------------------------------------------------
+class ReinitRun:
+    """Plain reinitialization: 128^3 cells on [-1,1]^3 (cell centres), 3
+    ghost layers, phi0 = (|x-c| - 0.5)(1 + 0.1 sin 3pi x sin 3pi y sin 3pi z)
+    with c ~ U[-0.1,0.1]^3 (splitmix64 seed), `steps` pseudo-steps of
+    HJ-WENO5 -> reinitialization RHS (sign from the current phi) -> TVD-RK3
+    with dt = 0.5 dx, linear extrapolation into the ghost cells after every
+    stage.  dropin=True issues the reference's call sequence through the
+    drop-in entry points; otherwise the fused reinitialization stage."""
+
+    def __init__(self, n=128, seed=1, precision="f64", dropin=True, steps=50):
+        self.grid = Grid((n, n, n), lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0), centred=True)
+        self.solver = Solver(self.grid, precision, bc="linear")
+        phi0, self.centre = phi_perturbed_sphere(self.grid, seed)
+        s = self.solver
+        self.phi = s.field(phi0)
+        s.fill_ghosts(self.phi)
+        self.work = (s.field(), s.field())
+        self.dt = 0.5 * self.grid.dx[0]
+        self.steps = steps
+        self.chain = DropInChain(s) if dropin else None
+
+    def step(self):
+        if self.chain is not None:
+            self.chain.rk3(self.phi, self.work, self.chain.rhs_reinit, self.dt)
+        else:
+            self.solver.rk3_fused(self.phi, self.work, self.solver.stage_reinit, self.dt)
+
+    def run(self):
+        for _ in range(self.steps):
+            self.step()
+        return self.phi
