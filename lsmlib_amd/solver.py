@@ -87,8 +87,15 @@ def phi_perturbed_sphere(grid: Grid, seed: int):
     return (d - 0.5) * (1 + 0.1 * np.sin(3 * np.pi * x) * np.sin(3 * np.pi * y) * np.sin(3 * np.pi * z)), c
 
 
-def phi_spheres_min(grid: Grid, seed: int, count: int, centre_range, radius_range, chunk=16):
-    """min over seeded spheres of (|x - c_l| - r_l)."""
+def phi_spheres_min(grid: Grid, seed: int, count: int, centre_range, radius_range, chunk=16, threads=8):
+    """min over seeded spheres of (|x - c_l| - r_l), in FP64 on the host.
+
+    Work is split into (k, j) tiles of `chunk` x `chunk` lines.  A sphere is
+    skipped on a tile when a lower bound of its value there,
+    sqrt(dz_min^2 + dy_min^2) - r, is not below the tile's current maximum:
+    rounding is monotone in each operation, so a skipped sphere could not
+    lower any value and the result is bitwise the unpruned minimum."""
+    import concurrent.futures as cf
     r = SplitMix64(seed)
     spheres = []
     for _ in range(count):
@@ -96,13 +103,26 @@ def phi_spheres_min(grid: Grid, seed: int, count: int, centre_range, radius_rang
         spheres.append((c, r.uniform(*radius_range)))
     ax = [grid.axis(d) for d in range(3)]
     out = np.full(grid.shape, np.inf)
-    for (c, rad) in spheres:
-        dx2 = (ax[0] - c[0]) ** 2
-        dy2 = (ax[1] - c[1]) ** 2
-        dz2 = (ax[2] - c[2]) ** 2
-        for k0 in range(0, grid.shape[0], chunk):
-            blk = np.sqrt(dz2[k0:k0 + chunk, None, None] + dy2[None, :, None] + dx2[None, None, :]) - rad
-            np.minimum(out[k0:k0 + chunk], blk, out=out[k0:k0 + chunk])
+    nz, ny = grid.shape[0], grid.shape[1]
+
+    def nearest(a, lo, hi, c):
+        seg = a[lo:hi]
+        return 0.0 if seg[0] <= c <= seg[-1] else min(abs(seg[0] - c), abs(seg[-1] - c))
+
+    def tile(k0, j0):
+        o = out[k0:k0 + chunk, j0:j0 + chunk]
+        for (c, rad) in spheres:
+            dzm = nearest(ax[2], k0, k0 + chunk, c[2])
+            dym = nearest(ax[1], j0, j0 + chunk, c[1])
+            if math.sqrt(dzm * dzm + dym * dym) - rad >= o.max():
+                continue
+            dz2 = (ax[2][k0:k0 + chunk] - c[2]) ** 2
+            dy2 = (ax[1][j0:j0 + chunk] - c[1]) ** 2
+            dx2 = (ax[0] - c[0]) ** 2
+            np.minimum(o, np.sqrt(dz2[:, None, None] + dy2[None, :, None] + dx2[None, None, :]) - rad, out=o)
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda kj: tile(*kj), [(k0, j0) for k0 in range(0, nz, chunk) for j0 in range(0, ny, chunk)]))
     return out, spheres
 
 
@@ -435,6 +455,202 @@ class ReinitRun:
             self.chain.rk3(self.phi, self.work, self.chain.rhs_reinit, self.dt)
         else:
             self.solver.rk3_fused(self.phi, self.work, self.solver.stage_reinit, self.dt)
+
+    def run(self):
+        for _ in range(self.steps):
+            self.step()
+        return self.phi
+
+
+# ---------------------------------------------------------------------------
+# Periodic motion workload: advection + normal speed with a curvature term
+# ---------------------------------------------------------------------------
+def periodic_velocity(grid: Grid):
+    """u = 0.5 (sin 2 pi y, sin 2 pi z, sin 2 pi x) over the ghost box."""
+    x, y, z = grid.mesh()
+    return 0.5 * np.sin(2 * np.pi * y), 0.5 * np.sin(2 * np.pi * z), 0.5 * np.sin(2 * np.pi * x)
+
+
+def curvature_corrected_dt(dt_hyp: float, grid: Grid, b: float, cfl: float) -> float:
+    """dt = cfl / (cfl/dt_hyp + 2 b sum 1/dx^2): the parabolic correction of
+    examples/curvature_driven_motion/curvature_model3d.c:95-98, 260-266."""
+    corr = 2.0 * b * sum(1.0 / (h * h) for h in grid.dx)
+    return cfl / (cfl / dt_hyp + corr)
+
+
+class PeriodicMotionRun:
+    """256^3 on [0,1]^3 with periodic ghost cells.  phi0 = min over 32 seeded
+    spheres (centres U[0.15,0.85]^3, radii U[0.04,0.12]).  phi_t + u.grad phi
+    + V_n |grad phi| = 0 with u = 0.5 (sin 2pi y, sin 2pi z, sin 2pi x) and
+    V_n = 1 - b kappa, b = 0.02, kappa the second-order mean curvature.  Each
+    RK stage issues, through the drop-in entry points:
+      CENTRAL_GRAD_ORDER2 -> GRADIENT_MAGNITUDE -> COMPUTE_MEAN_CURVATURE_ORDER2,
+      UPWIND_HJ_WENO5 -> ZERO_OUT -> ADD_ADVECTION_TERM,
+      HJ_WENO5 -> ADD_CONST_NORMAL_VEL_TERM (V = 1),
+      ADD_CONST_PRECOMPUTED_CURV_TERM (b), TVD_RK3_STAGEn, periodic fill.
+    dt: min of COMPUTE_STABLE_ADVECTION_DT and COMPUTE_STABLE_CONST_NORMAL_VEL_DT
+    (from phi^n's WENO5 derivatives) at CFL 0.5, with the parabolic correction
+    of the curvature term."""
+
+    B = 0.02
+    CFL = 0.5
+
+    def __init__(self, n=256, seed=2, precision="f64", steps=100):
+        self.grid = g = Grid((n, n, n))
+        self.solver = s = Solver(g, precision, bc="periodic")
+        phi0, self.spheres = phi_spheres_min(g, seed, 32, (0.15, 0.85), (0.04, 0.12))
+        self.phi = s.field(phi0)
+        s.fill_ghosts(self.phi)
+        self.vel = [s.field(v) for v in periodic_velocity(g)]
+        self.work = (s.field(), s.field())
+        self.kappa, self.gmag = s.field(), s.field()
+        self.cg = [s.field() for _ in range(3)]
+        self.chain = DropInChain(s)
+        self.steps = steps
+        self.dts = []
+
+    def rhs(self, u):
+        g, a, ch = self.grid, self.solver.lib.api, self.chain
+        gb, fb, G, C = g.gb, g.fb, ch.grads, self.cg
+        a.lsm3dcentralgradorder2(C[0], C[1], C[2], gb, u, gb, fb, *g.dx)
+        a.lsm3dgradientmagnitude(C[0], C[1], C[2], self.gmag, gb, fb)
+        a.lsm3dcomputemeancurvatureorder2(self.kappa, gb, u, gb, C[0], C[1], C[2], self.gmag, gb, fb, *g.dx)
+        ch.rhs_advect(u, vel=self.vel)
+        a.lsm3dhjweno5(G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, u, gb, ch.d1, gb, fb, *g.dx)
+        a.lsm3daddconstnormalveltermtolserhs(ch.rhs, gb, G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, 1.0, fb)
+        a.lsm3daddconstprecomputedcurvtermtolserhs(ch.rhs, gb, self.kappa, gb, self.gmag, gb, self.B, fb)
+
+    def stable_dt(self):
+        g, a, ch = self.grid, self.solver.lib.api, self.chain
+        gb, fb, G = g.gb, g.fb, ch.grads
+        dt_adv = a.lsm3dcomputestableadvectiondt(None, self.vel[0], self.vel[1], self.vel[2], gb, fb, *g.dx, self.CFL)
+        a.lsm3dhjweno5(G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, self.phi, gb, ch.d1, gb, fb, *g.dx)
+        dt_n = a.lsm3dcomputestableconstnormalveldt(None, 1.0, G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, fb,
+                                                    *g.dx, self.CFL)
+        return curvature_corrected_dt(min(dt_adv, dt_n), g, self.B, self.CFL)
+
+    def step(self):
+        dt = self.stable_dt()
+        self.dts.append(dt)
+        self.chain.rk3(self.phi, self.work, lambda u: self.rhs(u), dt)
+
+    def run(self):
+        for _ in range(self.steps):
+            self.step()
+        return self.phi
+
+
+# ---------------------------------------------------------------------------
+# Narrow-band workload
+# ---------------------------------------------------------------------------
+class NarrowBandRun:
+    """Narrow-band normal motion (V_n = 1) on n^3 nodes of [0,1]^3.  phi0 =
+    min over 64 seeded spheres (centres U[0.1,0.9]^3, radii U[0.03,0.1]).
+    Band: |phi| < width = 6 dx at level 0, outer layer from width_inner = 3 dx,
+    levels 0..3, boundary layers marked as the reference's local driver marks
+    them (examples/curvature_driven_motion/curvature_model3d_local.c:190-229;
+    marks fb 124, D2 125, D1 126, gb 127).  The band is rebuilt every
+    `rebuild_every` steps.  One step, all through the drop-in entry points:
+      COPY_DATA_LOCAL phi -> both stage fields over levels 0..3 (the stencils
+        read frozen phi^n at levels 1..3),
+      HJ_WENO5_LOCAL(phi) -> COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL (CFL 0.5),
+      three stages of: HJ_WENO5_LOCAL (stage 1 reuses the derivatives above)
+        -> ZERO_OUT_LOCAL -> ADD_CONST_NORMAL_VEL_TERM_LOCAL (V = 1)
+        -> MULTIPLY_CUT_OFF_LSE_RHS_LOCAL(beta = width_inner, gamma = width)
+        -> TVD_RK3_STAGEn_LOCAL -> signed linear extrapolation fill."""
+
+    MARKS = (124, 125, 126, 127)  # fb, D2, D1, gb
+    CFL = 0.5
+
+    def __init__(self, n=512, seed=3, precision="f64", steps=200, rebuild_every=10, level=3, phi0=None):
+        import torch
+        self.grid = g = Grid((n, n, n))
+        self.solver = s = Solver(g, precision, bc="signed_linear")
+        if phi0 is None:
+            phi0, self.spheres = phi_spheres_min(g, seed, 64, (0.1, 0.9), (0.03, 0.1))
+        self.phi = s.field(phi0)
+        del phi0
+        s.fill_ghosts(self.phi)
+        self.work = (s.field(), s.field())
+        self.grads = [s.field() for _ in range(6)]
+        self.rhs, self.d1 = s.field(), s.field()
+        self.npts = g.shape[0] * g.shape[1] * g.shape[2]
+        dev = s.device
+        self.nb = torch.zeros(g.shape, dtype=torch.uint8, device=dev)
+        self.ix, self.iy, self.iz, self.outer = (torch.zeros(self.npts, dtype=torch.int32, device=dev)
+                                                  for _ in range(4))
+        self.level = level
+        self.n_lo = np.zeros(level + 1, np.int32)
+        self.n_hi = np.zeros(level + 1, np.int32)
+        self.width, self.width_inner = 6 * g.dx[0], 3 * g.dx[0]
+        self.steps, self.rebuild_every = steps, rebuild_every
+        self.nsteps = 0
+        self.dts, self.band_sizes = [], []
+        self.api = s.lib.api
+        self.on_rebuild = None  # callable(run) after each rebuild, e.g. to record the band
+
+    def boxes(self):
+        n, gb = self.grid.n, self.grid.gb
+        d2 = (-1, n[0], -1, n[1], -1, n[2])
+        d1 = (-2, n[0] + 1, -2, n[1] + 1, -2, n[2] + 1)
+        return d2, d1, gb
+
+    def rebuild(self):
+        a, g = self.api, self.grid
+        gb = g.gb
+        self.outer_bounds = a.lsm3ddeterminenarrowband(
+            self.phi, gb, self.nb, gb, self.ix, self.iy, self.iz, 0, self.npts - 1, self.n_lo, self.n_hi,
+            self.outer, 0, self.npts - 1, None, None, None, None, self.width, self.width_inner, self.level)
+        self.solver.lib.check()
+        if (self.n_hi < self.n_lo).any() or (self.n_lo < 0).any():
+            raise LsmHipError("empty narrow-band level: n_lo=%s n_hi=%s" % (self.n_lo, self.n_hi))
+        for box, mark in zip(self.boxes(), self.MARKS[1:]):
+            a.lsm3dmarknarrowbandboundarylayer(self.nb, gb, box, mark)
+        self.band_sizes.append(int(self.n_hi[0] - self.n_lo[0] + 1))
+        if self.on_rebuild is not None:
+            self.on_rebuild(self)
+
+    def _weno(self, u):
+        a, g, G = self.api, self.grid, self.grads
+        gb = g.gb
+        lv = [int(v) for pair in zip(self.n_lo, self.n_hi) for v in pair]
+        a.lsm3dhjweno5local(G[0], G[1], G[2], gb, G[3], G[4], G[5], gb, u, gb, self.d1, gb, *g.dx,
+                            self.ix, self.iy, self.iz, *lv[:8], self.nb, gb, self.MARKS[0], self.MARKS[2])
+
+    def step(self):
+        a, g, G = self.api, self.grid, self.grads
+        gb = g.gb
+        if self.nsteps % self.rebuild_every == 0:
+            self.rebuild()
+        lo0, hi0, hi3 = int(self.n_lo[0]), int(self.n_hi[0]), int(self.n_hi[self.level])
+        band = (self.ix, self.iy, self.iz)
+        mfb = self.MARKS[0]
+        phi, (p1, p2) = self.phi, self.work
+        for w in (p1, p2):
+            a.lsm3dcopydatalocal(w, gb, phi, gb, *band, lo0, hi3)
+        self._weno(phi)
+        dt = a.lsm3dcomputestableconstnormalveldtlocal(None, 1.0, G[0], G[1], G[2], gb, G[3], G[4], G[5], gb,
+                                                       *g.dx, self.CFL, *band, lo0, hi0, self.nb, gb, mfb)
+        self.dts.append(dt)
+        for st, (src, out) in enumerate(((phi, p1), (p1, p2), (p2, phi)), start=1):
+            if st > 1:
+                self._weno(src)
+            a.lsm3dzerooutlevelseteqnrhslocal(self.rhs, gb, *band, lo0, hi0)
+            a.lsm3daddconstnormalveltermtolserhslocal(self.rhs, gb, G[0], G[1], G[2], gb, G[3], G[4], G[5], gb,
+                                                      1.0, *band, lo0, hi0, self.nb, gb, mfb)
+            a.lsm3dmultiplycutofflserhslocal(src, self.rhs, gb, *band, lo0, hi0, self.nb, gb, mfb,
+                                             self.width_inner, self.width)
+            if st == 1:
+                a.lsm3dtvdrk3stage1local(out, gb, phi, gb, self.rhs, gb, dt, *band, lo0, hi0, self.nb, gb, mfb)
+            elif st == 2:
+                a.lsm3dtvdrk3stage2local(out, gb, p1, gb, phi, gb, self.rhs, gb, dt, *band, lo0, hi0, self.nb, gb,
+                                         mfb)
+            else:
+                a.lsm3dtvdrk3stage3local(out, gb, p2, gb, phi, gb, self.rhs, gb, dt, *band, lo0, hi0, self.nb, gb,
+                                         mfb)
+            self.solver.fill_ghosts(out)
+        self.solver.lib.check()
+        self.nsteps += 1
 
     def run(self):
         for _ in range(self.steps):
