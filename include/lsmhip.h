@@ -90,6 +90,28 @@ int lsmhip_deform_velocity_fill(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_R
 /* k planes a fused-stage block marches through (default 64). */
 void lsmhip_set_fused_kchunk(int planes);
 
+/* ---- reverse mode of the normal-motion stage (for gradient-based inverse problems) ----
+ * Reverse of lsmhip_weno5_stage_normal for stage 1, 2, 3 (TVD-RK3) or 22
+ * (TVD-RK2 stage 2): with out = a u_c + b (u_s + dt L(u_s)), L = -V sqrt(G),
+ * given lam_out = dJ/d out on fb it adds b lam_out + (b dt dL/du_s)^T lam_out
+ * to lam_us (ghost box), a lam_out to lam_uc (fb; unused for stage 1, may be
+ * NULL) and -b dt sqrt(G) lam_out to lam_vn (fb; only with vn_field, may be
+ * NULL).  scratch: 21 * |gb| reals.  Derivatives are those of the implemented
+ * WENO5/Godunov functions (argmax of the WENO eps, the selected Godunov branch). */
+int lsmhip_weno5_stage_normal_adjoint(LSMLIB_REAL *lam_us, LSMLIB_REAL *lam_uc, LSMLIB_REAL *lam_vn,
+                                      const LSMLIB_REAL *lam_out, const LSMLIB_REAL *phi_stage, LSMHIP_BOX_ARGS(gb),
+                                      LSMHIP_BOX_ARGS(fb), int stage, LSMLIB_REAL vn, const LSMLIB_REAL *vn_field,
+                                      LSMLIB_REAL dt, LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                      LSMLIB_REAL *scratch);
+/* Adjoint of the periodic ghost fill: adds each ghost entry to the interior
+ * cell it copies, then zeroes the ghosts. */
+int lsmhip_periodic_fill_adjoint(LSMLIB_REAL *lam, LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb));
+/* J = sum_fb (H_eps(phi) - H_eps(target))^2 dx dy dz in FP64, and dJ/dphi into
+ * grad (gb, zero off fb) when grad is not NULL. */
+int lsmhip_heaviside_mismatch(double *loss, LSMLIB_REAL *grad, const LSMLIB_REAL *phi, const LSMLIB_REAL *target,
+                              LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), double dx, double dy, double dz,
+                              double epsilon);
+
 /* ---- diagnostics ---- */
 /* Smoothed volume of {phi < 0} (the reference's lsm3dVolumeRegionPhiLessThanZero
  * integrand) accumulated and returned in FP64 whatever LSMLIB_REAL is. */
