@@ -273,7 +273,7 @@ class ShapeMatching:
         phi0 = (self.phi0 if phi0 is None else phi0).detach().clone().requires_grad_(True)
         L = self.loss(a, phi0)
         ga, gphi = t.autograd.grad(L, (a, phi0))
-        return float(L), ga, gphi
+        return float(L.detach()), ga, gphi
 
     def value(self, a, phi0):
         Evolve, Mismatch = autograd_functions()
