@@ -214,9 +214,11 @@ def fourier_modes(seed: int, modes=8):
 
 
 class SpeedField:
-    """g(x) = sum_k a_k cos(2 pi k.x + theta_k) / sqrt(sum_k (a_k)^2 / 2) ... kept
-    as an explicit torch computation so that autograd carries dJ/dV to dJ/da:
-    V = 1 + 0.3 tanh(c * g) with c fixed from the seed's amplitudes."""
+    """V(x) = 1 + 0.3 tanh(c g(x)), g(x) = sum_k a_k cos(2 pi k.x + theta_k)
+    over k in {1..8}^3; the phases theta are fixed by the seed, the amplitudes a
+    are the parameters (initially the seed's), and c = 1/sqrt(sum a0^2 / 2)
+    scales the seed's g to unit RMS.  Evaluated in torch so that autograd
+    carries dJ/dV to dJ/da."""
 
     def __init__(self, grid: Grid, seed: int, modes=8, device="cuda"):
         import torch
