@@ -148,9 +148,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=64)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box of this build has one MI355X and the path is not sharded "
-              "across GPUs (the slab decomposition is described in DESIGN.md); run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the level-set path runs on a single MI355X here and is not sharded "
+              "across GPUs; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -6,11 +6,15 @@
 //
 // Memory layout and traffic: phi_stage, phi_n and the output share one ghost
 // box.  A block owns a TX x TY tile of the (i,j) plane and marches along k
-// through a chunk of KC planes, keeping a ring of 7 planes (the WENO5 reach
-// is +-3) of phi_stage with their x/y halo in LDS; each plane is read from
-// global memory once per block.  Neighbouring tiles share halos through L2:
-// blocks are numbered so that each XCD (blocks are dealt round-robin to the 8
-// XCDs) processes one contiguous range of tiles.
+// through a chunk of KC planes.  Each thread keeps the seven values of its own
+// column around plane k (the WENO5 reach is +-3) in registers and shifts them
+// as k advances, so the z stencil needs neither LDS nor index arithmetic; the
+// current plane with its 3-cell x/y halo is staged in LDS for the x and y
+// stencils (interior from the threads' windows, the halo loaded by one thread
+// per halo cell).  Each plane is read from global memory once per block;
+// neighbouring tiles share halos through L2: blocks are numbered so that each
+// XCD (blocks are dealt round-robin to the 8 XCDs) processes one contiguous
+// range of tiles.
 //
 // The per-cell arithmetic is that of the drop-in chain (stencil.hpp), so in
 // the FP64 build a stage is bitwise equal to
@@ -25,7 +29,6 @@ namespace {
 
 constexpr int TX = 64, TY = 8, H = 3;
 constexpr int PX = TX + 2 * H, PY = TY + 2 * H, PLANE = PX * PY;
-constexpr int RING = 2 * H + 1;
 
 enum RhsKind { kNormalConst = 0, kNormalField = 1, kAdvectField = 2, kAdvectDeform = 3, kReinit = 4 };
 
@@ -79,10 +82,27 @@ __device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, re
   return real_t(0.5) * (uc + us + dt * r);  // TVD-RK2 stage 2
 }
 
+// Position (px, py) in the PX x PY plane tile of halo element h (0 <= h < NHALO):
+// three full rows above, three below, then three columns left and right of
+// the TY interior rows.
+constexpr int NHALO = PLANE - TX * TY;
+__device__ __forceinline__ void halo_pos(int h, int& px, int& py) {
+  if (h < H * PX) {
+    py = h / PX;
+    px = h % PX;
+  } else if (h < 2 * H * PX) {
+    py = H + TY + (h - H * PX) / PX;
+    px = (h - H * PX) % PX;
+  } else {
+    const int t = h - 2 * H * PX, c = t % (2 * H);
+    py = H + t / (2 * H);
+    px = c < H ? c : TX + c;
+  }
+}
+
 template <int RHS, int STAGE>
 __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  real_t* ring = reinterpret_cast<real_t*>(smem_raw);
+  __shared__ real_t plane[PLANE];
   const int tid = threadIdx.y * TX + threadIdx.x;
 
   // XCD-aware tile order: block b runs on XCD b % 8; give each XCD a contiguous tile range.
@@ -96,46 +116,55 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
   const int kb = a.fb.klo + tk * a.kc;
   const int ke = min(kb + a.kc - 1, a.fb.khi);
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
-
   const Box gb = a.gb;
-  // Staging slots of this thread in a plane tile (the same for every plane):
-  // LDS element e = tid + r*TX*TY and its in-plane global offset, or -1 when
-  // the element lies outside the ghost box (never read by an active cell).
-  constexpr int NSLOT = (PLANE + TX * TY - 1) / (TX * TY);
-  int soff[NSLOT];
-#pragma unroll
-  for (int r = 0; r < NSLOT; r++) {
-    const int e = tid + r * TX * TY;
-    const int gi = i0 - H + e % PX, gj = j0 - H + e / PX;
-    soff[r] = (e < PLANE && gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
-                  ? (gi - gb.ilo) + a.sy * (gj - gb.jlo)
-                  : -1;
+  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
+  // own column: needed by neighbours whenever it lies in the ghost box
+  const bool own_in = i <= gb.ihi && j <= gb.jhi;
+  const long long own = (long long)(i - gb.ilo) + (long long)a.sy * (j - gb.jlo);
+  // halo element of this thread (the same slot in every plane), -1 when none or outside the ghost box
+  long long hoff = -1;
+  int hslot = 0;
+  if (tid < NHALO) {
+    int px, py;
+    halo_pos(tid, px, py);
+    const int gi = i0 - H + px, gj = j0 - H + py;
+    hslot = py * PX + px;
+    if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
+      hoff = (long long)(gi - gb.ilo) + (long long)a.sy * (gj - gb.jlo);
   }
-  auto load_plane = [&](int kp) {
-    if (kp < gb.klo || kp > gb.khi) return;  // uniform across the block
-    real_t* dst = ring + ((kp - kb + 3 * RING) % RING) * PLANE;
-    const real_t* src = a.us + a.sz * (kp - gb.klo);
-#pragma unroll
-    for (int r = 0; r < NSLOT; r++)
-      if (soff[r] >= 0) dst[tid + r * TX * TY] = src[soff[r]];
-  };
-  for (int kp = kb - H; kp < kb + H; kp++) load_plane(kp);
-
   const int lx = threadIdx.x + H, ly = threadIdx.y + H;
+  const real_t* us = a.us;
+  const long long sz = a.sz;
+
+  // z window of the own column: zw[s+3] = u(i, j, k+s)
+  real_t zw[7];
+#pragma unroll
+  for (int s = 0; s < 6; s++) zw[s] = own_in ? us[own + sz * (kb - H + s - gb.klo)] : real_t(0);
+  real_t ta0 = 0, tb0 = 0, ta1 = 0, tb1 = 0;
+  if (RHS == kAdvectDeform && active) {
+    ta0 = a.ta[0][i - gb.ilo];
+    tb0 = a.tb[0][i - gb.ilo];
+    ta1 = a.ta[1][j - gb.jlo];
+    tb1 = a.tb[1][j - gb.jlo];
+  }
+
   for (int k = kb; k <= ke; k++) {
-    load_plane(k + H);
+    const long long zoff = sz * (k - gb.klo);
+    zw[6] = own_in ? us[own + zoff + H * sz] : real_t(0);
+    const real_t hv = hoff >= 0 ? us[hoff + zoff] : real_t(0);
+    __syncthreads();  // the previous plane's reads are done
+    plane[ly * PX + lx] = zw[3];
+    if (tid < NHALO) plane[hslot] = hv;
     __syncthreads();
     if (active) {
-      const real_t* pc = ring + ((k - kb + 3 * RING) % RING) * PLANE;
       real_t fx[7], fy[7], fz[7];
 #pragma unroll
       for (int s = -H; s <= H; s++) {
-        fx[s + 3] = pc[ly * PX + lx + s];
-        fy[s + 3] = pc[(ly + s) * PX + lx];
-        fz[s + 3] = ring[((k + s - kb + 3 * RING) % RING) * PLANE + ly * PX + lx];
+        fx[s + 3] = plane[ly * PX + lx + s];
+        fy[s + 3] = plane[(ly + s) * PX + lx];
+        fz[s + 3] = zw[s + 3];
       }
-      const int off = (i - gb.ilo) + a.sy * (j - gb.jlo) + a.sz * (k - gb.klo);
+      const long long off = own + zoff;
       real_t r;
       if (RHS == kAdvectField || RHS == kAdvectDeform) {
         real_t u, v, w;
@@ -144,7 +173,10 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
           v = a.vy[off];
           w = a.vz[off];
         } else {
-          deform_velocity(a.ta, a.tb, i - gb.ilo, j - gb.jlo, k - gb.klo, a.tfac, u, v, w);
+          const real_t ta2 = a.ta[2][k - gb.klo], tb2 = a.tb[2][k - gb.klo], c = a.tfac;
+          u = real_t(2) * ta0 * tb1 * tb2 * c;
+          v = -(tb0 * ta1 * tb2 * c);
+          w = -(tb0 * tb1 * ta2 * c);
         }
         const real_t px = fabs(u) < kZeroTol ? real_t(0) : weno_up(fx, a.ix, u > real_t(0));
         const real_t py = fabs(v) < kZeroTol ? real_t(0) : weno_up(fy, a.iy, v > real_t(0));
@@ -167,17 +199,18 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
       const real_t uc = STAGE == 1 ? fx[3] : a.uc[off];
       a.out[off] = rk_combine<STAGE>(uc, fx[3], a.dt, r);
     }
-    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 6; s++) zw[s] = zw[s + 1];
   }
 }
 
 template <int RHS>
-bool launch_rhs(int stage, const FusedArgs& a, int nblocks, size_t lds) {
+bool launch_rhs(int stage, const FusedArgs& a, int nblocks) {
   switch (stage) {
-    case 1: hipLaunchKernelGGL((fused_stage_kernel<RHS, 1>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
-    case 2: hipLaunchKernelGGL((fused_stage_kernel<RHS, 2>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
-    case 3: hipLaunchKernelGGL((fused_stage_kernel<RHS, 3>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
-    case 22: hipLaunchKernelGGL((fused_stage_kernel<RHS, 22>), dim3(nblocks), dim3(TX, TY), lds, stream(), a); break;
+    case 1: hipLaunchKernelGGL((fused_stage_kernel<RHS, 1>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
+    case 2: hipLaunchKernelGGL((fused_stage_kernel<RHS, 2>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
+    case 3: hipLaunchKernelGGL((fused_stage_kernel<RHS, 3>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
+    case 22: hipLaunchKernelGGL((fused_stage_kernel<RHS, 22>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
     default: return false;
   }
   return true;
@@ -202,14 +235,13 @@ int run(const char* who, FusedArgs a, int stage) {
   const long long nt = (long long)a.nti * a.ntj * a.ntk;
   const int per = (int)((nt + 7) / 8);
   const int nblocks = 8 * per;  // surplus blocks of the last XCD range exit at once
-  const size_t lds = sizeof(real_t) * PLANE * RING;
   bool ok = false;
   switch (a.rhs) {
-    case kNormalConst: ok = launch_rhs<kNormalConst>(stage, a, nblocks, lds); break;
-    case kNormalField: ok = launch_rhs<kNormalField>(stage, a, nblocks, lds); break;
-    case kAdvectField: ok = launch_rhs<kAdvectField>(stage, a, nblocks, lds); break;
-    case kAdvectDeform: ok = launch_rhs<kAdvectDeform>(stage, a, nblocks, lds); break;
-    case kReinit: ok = launch_rhs<kReinit>(stage, a, nblocks, lds); break;
+    case kNormalConst: ok = launch_rhs<kNormalConst>(stage, a, nblocks); break;
+    case kNormalField: ok = launch_rhs<kNormalField>(stage, a, nblocks); break;
+    case kAdvectField: ok = launch_rhs<kAdvectField>(stage, a, nblocks); break;
+    case kAdvectDeform: ok = launch_rhs<kAdvectDeform>(stage, a, nblocks); break;
+    case kReinit: ok = launch_rhs<kReinit>(stage, a, nblocks); break;
   }
   if (!ok) {
     set_error(kErrArg, "%s: stage must be 1, 2, 3 (TVD-RK3) or 22 (TVD-RK2 stage 2), got %d", who, stage);
