@@ -318,8 +318,7 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_normal_adjoint(real_t* lam_us, real_t* lam_
   A.sz = A.gb.nx() * A.gb.ny();
   hipLaunchKernelGGL(adj_weights_kernel, grid_for(A.fb), block2d(), 0, stream(), A);
   hipLaunchKernelGGL(adj_gather_kernel, grid_for(A.gb), block2d(), 0, stream(), A);
-  finish(who);
-  return 0;
+  return finish(who) ? 0 : -1;
 }
 
 // Adjoint of lsmhip_fill_ghosts(kind = periodic, all faces): adds every ghost
@@ -336,8 +335,7 @@ LSMHIP_EXPORT int lsmhip_periodic_fill_adjoint(real_t* lam, LSMHIP_BOXP(gb), LSM
   }
   hipLaunchKernelGGL(fold_kernel, grid_for(f), block2d(), 0, stream(), lam, g, f, g.nx(), g.nx() * g.ny());
   hipLaunchKernelGGL(zero_ghosts_kernel, grid_for(g), block2d(), 0, stream(), lam, g, f, g.nx(), g.nx() * g.ny());
-  finish(who);
-  return 0;
+  return finish(who) ? 0 : -1;
 }
 
 // J = sum over fb of (H_eps(phi) - H_eps(target))^2 dx dy dz, accumulated in
@@ -356,7 +354,7 @@ LSMHIP_EXPORT int lsmhip_heaviside_mismatch(double* loss, real_t* grad, const re
   if (grad) {
     hipLaunchKernelGGL(mismatch_grad_kernel, grid_for(g), block2d(), 0, stream(), grad, phi, target, g, f, g.nx(),
                        g.nx() * g.ny(), epsilon, dx * dy * dz);
-    finish(who);
+    if (!finish(who)) return -1;
   }
   return 0;
 }

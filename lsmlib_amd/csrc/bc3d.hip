@@ -82,6 +82,122 @@ void face(const char* who, real_t* phi, const Box& gb, const Box& fb, int idx, i
   finish(who);
 }
 
+// All six faces in one launch.  Applying faces x_lo, x_hi, y_lo, y_hi, z_lo,
+// z_hi in turn leaves in every ghost cell the value written by the face of its
+// last ghost axis (z before y before x), computed from cells whose coordinate
+// on that axis is the edge or next-inner layer; those cells' own values come
+// from the faces of the remaining ghost axes, all applied earlier.  So the final
+// value of a ghost cell is that recursion evaluated from interior cells only,
+// with the face kernel's operations; one thread per ghost cell computes it
+// and no thread reads a ghost cell.
+struct Shell {
+  Box gb, fb;
+  long long nzr, nyr;  // cells in the z slabs (k ghost) and in the y slabs (k interior, j ghost)
+  long long total;
+  int kind;
+};
+
+__device__ __forceinline__ real_t extrap(real_t pb, real_t pin, int dist, int kind) {
+  if (kind == kCopy) return pb;
+  const real_t slope = kind == kLinear ? pb - pin : copysign(real_t(1), pb) * fabs(pb - pin);
+  return pb + slope * real_t(dist);
+}
+
+// Value of cell (i, j, k) once the faces of axis 0 (v0: j, k interior), of
+// axes 0..1 (v1: k interior) and of all three axes (v2) have been applied.
+__device__ __forceinline__ real_t shell_v0(const View<real_t>& P, const Shell& S, int i, int j, int k) {
+  const Box& f = S.fb;
+  if (i >= f.ilo && i <= f.ihi) return P(i, j, k);
+  const bool up = i > f.ihi;
+  if (S.kind == kPeriodic) return P(up ? i - f.nx() : i + f.nx(), j, k);
+  const int e = up ? f.ihi : f.ilo, n = up ? f.ihi - 1 : f.ilo + 1;
+  return extrap(P(e, j, k), S.kind == kCopy ? real_t(0) : P(n, j, k), up ? i - e : e - i, S.kind);
+}
+__device__ __forceinline__ real_t shell_v1(const View<real_t>& P, const Shell& S, int i, int j, int k) {
+  const Box& f = S.fb;
+  if (j >= f.jlo && j <= f.jhi) return shell_v0(P, S, i, j, k);
+  const bool up = j > f.jhi;
+  if (S.kind == kPeriodic) return shell_v0(P, S, i, up ? j - f.ny() : j + f.ny(), k);
+  const int e = up ? f.jhi : f.jlo, n = up ? f.jhi - 1 : f.jlo + 1;
+  const real_t pb = shell_v0(P, S, i, e, k);
+  return extrap(pb, S.kind == kCopy ? real_t(0) : shell_v0(P, S, i, n, k), up ? j - e : e - j, S.kind);
+}
+__device__ __forceinline__ real_t shell_v2(const View<real_t>& P, const Shell& S, int i, int j, int k) {
+  const Box& f = S.fb;
+  if (k >= f.klo && k <= f.khi) return shell_v1(P, S, i, j, k);
+  const bool up = k > f.khi;
+  if (S.kind == kPeriodic) return shell_v1(P, S, i, j, up ? k - f.nz() : k + f.nz());
+  const int e = up ? f.khi : f.klo, n = up ? f.khi - 1 : f.klo + 1;
+  const real_t pb = shell_v1(P, S, i, j, e);
+  return extrap(pb, S.kind == kCopy ? real_t(0) : shell_v1(P, S, i, j, n), up ? k - e : e - k, S.kind);
+}
+
+__global__ void __launch_bounds__(256) shell_kernel(View<real_t> P, Shell S) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.total) return;
+  const Box &g = S.gb, &f = S.fb;
+  const int nxg = g.nx(), nyg = g.ny();
+  int i, j, k;
+  if (t < S.nzr) {  // k ghost, i and j over the ghost box
+    const long long plane = (long long)nxg * nyg;
+    const int kk = (int)(t / plane);
+    const int below = f.klo - g.klo;
+    k = kk < below ? g.klo + kk : f.khi + 1 + (kk - below);
+    const long long r = t % plane;
+    j = g.jlo + (int)(r / nxg);
+    i = g.ilo + (int)(r % nxg);
+  } else if (t < S.nzr + S.nyr) {  // k interior, j ghost, i over the ghost box
+    const long long u = t - S.nzr;
+    const int ngy = (f.jlo - g.jlo) + (g.jhi - f.jhi);
+    const long long row = (long long)nxg * ngy;
+    k = f.klo + (int)(u / row);
+    const long long r = u % row;
+    const int jj = (int)(r / nxg);
+    const int below = f.jlo - g.jlo;
+    j = jj < below ? g.jlo + jj : f.jhi + 1 + (jj - below);
+    i = g.ilo + (int)(r % nxg);
+  } else {  // k and j interior, i ghost
+    const long long u = t - S.nzr - S.nyr;
+    const int ngx = (f.ilo - g.ilo) + (g.ihi - f.ihi);
+    const long long row = (long long)ngx * f.ny();
+    k = f.klo + (int)(u / row);
+    const long long r = u % row;
+    j = f.jlo + (int)(r / ngx);
+    const int ii = (int)(r % ngx);
+    const int below = f.ilo - g.ilo;
+    i = ii < below ? g.ilo + ii : f.ihi + 1 + (ii - below);
+  }
+  P(i, j, k) = shell_v2(P, S, i, j, k);
+}
+
+// Checks of face() for all six faces, then one shell launch.
+int shell(const char* who, real_t* phi, const Box& gb, const Box& fb, int kind) {
+  if (gb.empty() || fb.empty()) return 0;
+  if (!check_box(gb, fb, who, "phi") || !check_size(gb, who)) return -1;
+  const int lo_f[3] = {fb.ilo, fb.jlo, fb.klo}, hi_f[3] = {fb.ihi, fb.jhi, fb.khi};
+  const int lo_g[3] = {gb.ilo, gb.jlo, gb.klo}, hi_g[3] = {gb.ihi, gb.jhi, gb.khi};
+  for (int idx = 0; idx < 6; idx++) {
+    const int d = idx / 2;
+    const int depth = idx % 2 ? hi_g[d] - hi_f[d] : lo_f[d] - lo_g[d];
+    if (depth <= 0) continue;
+    if ((kind == kLinear || kind == kSigned) && hi_f[d] == lo_f[d]) {
+      set_error(kErrBox, "%s: the layer next to face %d lies outside the ghost box", who, idx);
+      return -1;
+    }
+    if (kind == kPeriodic && hi_f[d] - lo_f[d] + 1 < depth) {
+      set_error(kErrBox, "%s: periodic fill needs at least %d interior cells normal to face %d", who, depth, idx);
+      return -1;
+    }
+  }
+  Shell S{gb, fb, 0, 0, 0, kind};
+  S.nzr = (long long)((fb.klo - gb.klo) + (gb.khi - fb.khi)) * gb.ny() * gb.nx();
+  S.nyr = (long long)fb.nz() * ((fb.jlo - gb.jlo) + (gb.jhi - fb.jhi)) * gb.nx();
+  S.total = S.nzr + S.nyr + (long long)fb.nz() * fb.ny() * ((fb.ilo - gb.ilo) + (gb.ihi - fb.ihi));
+  if (S.total == 0) return 0;
+  hipLaunchKernelGGL(shell_kernel, dim3((unsigned)((S.total + 255) / 256)), dim3(256), 0, stream(), view(phi, gb), S);
+  return finish(who) ? 0 : -1;
+}
+
 }  // namespace
 
 LSMHIP_EXPORT void lsm3dlinearextrapolation_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
@@ -137,6 +253,7 @@ LSMHIP_EXPORT int lsmhip_fill_ghosts(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(f
     set_error(kErrArg, "%s: boundary location index %d not in 0..9", who, bdry_location_idx);
     return -1;
   }
+  if (bdry_location_idx == 9) return shell(who, phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), kind);
   for (int f = first; f <= last; f++) face(who, phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), f, kind);
   return 0;
 }

@@ -247,8 +247,7 @@ int run(const char* who, FusedArgs a, int stage) {
     set_error(kErrArg, "%s: stage must be 1, 2, 3 (TVD-RK3) or 22 (TVD-RK2 stage 2), got %d", who, stage);
     return -1;
   }
-  finish(who);
-  return 0;
+  return finish(who) ? 0 : -1;
 }
 
 FusedArgs base(real_t* out, const real_t* us, const real_t* uc, const Box& gb, const Box& fb, real_t dt, real_t dx,

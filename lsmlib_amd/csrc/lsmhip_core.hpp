@@ -93,7 +93,7 @@ hipStream_t stream();
 bool synchronous();
 // Ends an entry point: records a launch error and, in synchronous mode (the
 // default, matching the reference's completion semantics), waits for the stream.
-void finish(const char* who);
+bool finish(const char* who);  // launch status, and completion in synchronous mode
 bool check_hip(hipError_t e, const char* who);
 // Device scratch owned by the library, grown on demand (never shrinks).
 void* workspace(size_t bytes);

@@ -46,9 +46,9 @@ bool check_hip(hipError_t e, const char* who) {
   return false;
 }
 
-void finish(const char* who) {
-  if (!check_hip(hipGetLastError(), who)) return;
-  if (g_sync) check_hip(hipStreamSynchronize(g_stream), who);
+bool finish(const char* who) {
+  if (!check_hip(hipGetLastError(), who)) return false;
+  return !g_sync || check_hip(hipStreamSynchronize(g_stream), who);
 }
 
 bool check_box(const Box& outer, const Box& inner, const char* who, const char* what) {

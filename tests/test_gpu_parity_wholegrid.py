@@ -66,3 +66,41 @@ def test_box_validation_reports_error(hip):
     assert code == 1 and "phi" in msg
     assert np.array_equal(res["pxp"], c.arrays["pxp"])  # untouched
     del before
+
+
+FACE_SYM = {0: "lsm3dlinearextrapolation", 1: "lsm3dsignedlinearextrapolation", 2: "lsm3dcopyextrapolation",
+            3: "lsmhip_periodic_face"}
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_fill_all_faces_in_one_launch(kind, prec):
+    """lsmhip_fill_ghosts(idx 9) fills the whole ghost shell in one launch; it
+    equals the six faces applied in order x_lo .. z_hi bit for bit (FP64:
+    against the oracle; FP32: against the library's own face kernels), from
+    random stale ghost values and asymmetric ghost widths."""
+    import ctypes
+    import torch
+    import lsmlib_amd
+    lib = lsmlib_amd.load(prec)
+    gb, fb = (-3, 12, -2, 9, -3, 7), (0, 9, 0, 6, 0, 4)
+    rng = np.random.default_rng(kind)
+    phi = rng.standard_normal(H.shape_of(gb))
+    box = [ctypes.POINTER(ctypes.c_int)] * 6
+    fill = lib.fn("lsmhip_fill_ghosts", [ctypes.c_void_p] + box + box + [ctypes.c_int, ctypes.c_int])
+    bx = lambda b: [ctypes.byref(ctypes.c_int(v)) for v in b]  # noqa: E731
+    dt = torch.float64 if prec == "f64" else torch.float32
+    d = torch.tensor(phi, dtype=dt, device="cuda")
+    assert fill(d.data_ptr(), *bx(gb), *bx(fb), 9, kind) == 0
+    lib.check()
+    got = d.cpu().numpy()
+    if prec == "f64":
+        want = phi.copy()
+        for idx in range(6):
+            getattr(H.oracle(), FACE_SYM[kind])(want, gb, fb, idx)
+    else:
+        e = torch.tensor(phi, dtype=dt, device="cuda")
+        for idx in range(6):
+            assert fill(e.data_ptr(), *bx(gb), *bx(fb), idx, kind) == 0
+        want = e.cpu().numpy()
+    assert np.array_equal(got, want)
