@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 timings and the accuracy check")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one device copy of 1 GiB before timing (the known-size kernel for HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the level-set path runs on a single MI355X here and is not sharded "
@@ -155,6 +157,12 @@ def main():
     import torch
     from lsmlib_amd.solver import DeformationRun
 
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        _sync()
+        del src, dst
     print("[bench] building the %d^3 deformation workload (FP32 fused)" % a.n, file=sys.stderr, flush=True)
     run32 = DeformationRun(a.n, "f32")
     for _ in range(a.warmup):

@@ -74,6 +74,19 @@ __device__ __forceinline__ float weno_up(const float f[7], float inv, bool vpos)
   return stencil::fastf::upwind(f, inv, vpos);
 }
 
+// Square root and reinitialization RHS of the fused stages: the reference's
+// operations in FP64; the hardware square root (about 1 ulp) in FP32.
+__device__ __forceinline__ double stage_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float stage_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double stage_reinit(double s, bool u0, double xp, double yp, double zp, double xm,
+                                               double ym, double zm, double dx_sq) {
+  return stencil::reinit_rhs(s, u0, xp, yp, zp, xm, ym, zm, dx_sq);
+}
+__device__ __forceinline__ float stage_reinit(float s, bool u0, float xp, float yp, float zp, float xm, float ym,
+                                              float zm, float dx_sq) {
+  return stencil::fastf::reinit_rhs(s, u0, xp, yp, zp, xm, ym, zm, dx_sq);
+}
+
 template <int STAGE>
 __device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, real_t r) {
   if (STAGE == 1) return uc + dt * r;
@@ -189,11 +202,11 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
         weno_pm(fz, a.iz, zp, zm);
         if (RHS == kReinit) {
           const real_t s = a.use_phi0 ? a.phi0[off] : fx[3];
-          r = stencil::reinit_rhs(s, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
+          r = stage_reinit(s, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
         } else {
           const real_t v = RHS == kNormalField ? a.vnf[off] : a.vn;
           r = real_t(0);
-          if (fabs(v) >= kZeroTol) r = r - v * sqrt(stencil::godunov_sq(v > real_t(0), xp, yp, zp, xm, ym, zm));
+          if (fabs(v) >= kZeroTol) r = r - v * stage_sqrt(stencil::godunov_sq(v > real_t(0), xp, yp, zp, xm, ym, zm));
         }
       }
       const real_t uc = STAGE == 1 ? fx[3] : a.uc[off];
@@ -204,13 +217,131 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
   }
 }
 
+// FP32 stages that need both one-sided derivatives (normal motion and
+// reinitialization).  The plus derivative at a cell and the minus derivative
+// at the next cell along an axis share one WENO5 window (stencil::fastf::win5),
+// so each window is formed once along z (the thread carries the window of
+// plane k+1 to the next plane) and along y (the windows of a tile row go
+// through LDS; the last row of the tile forms its neighbour's itself, one
+// wave in eight).  Along x both windows are formed in the lane: a neighbour's
+// window would come from the next lane, but the tile's last lane would then
+// form its own in a divergent branch, which costs the whole wave as much.
+using stencil::fastf::Win;
+
+// F is real_t (float): a template parameter so that the FP64 build, which never
+// launches this kernel, does not compile its FP32-only body.
+template <int RHS, int STAGE, class F>
+__global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
+  using namespace stencil::fastf;
+  __shared__ F plane[PLANE];
+  __shared__ Win ywin[TY][TX];
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int nt = a.nti * a.ntj * a.ntk;
+  const int b = blockIdx.x;
+  const int per = (nt + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= nt) return;
+  const int ti = t % a.nti, tj = (t / a.nti) % a.ntj, tk = t / (a.nti * a.ntj);
+  const int i0 = a.fb.ilo + ti * TX, j0 = a.fb.jlo + tj * TY;
+  const int kb = a.fb.klo + tk * a.kc;
+  const int ke = min(kb + a.kc - 1, a.fb.khi);
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const Box gb = a.gb;
+  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
+  const bool own_in = i <= gb.ihi && j <= gb.jhi;
+  const long long own = (long long)(i - gb.ilo) + (long long)a.sy * (j - gb.jlo);
+  long long hoff = -1;
+  int hslot = 0;
+  if (tid < NHALO) {
+    int px, py;
+    halo_pos(tid, px, py);
+    const int gi = i0 - H + px, gj = j0 - H + py;
+    hslot = py * PX + px;
+    if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
+      hoff = (long long)(gi - gb.ilo) + (long long)a.sy * (gj - gb.jlo);
+  }
+  const int lx = threadIdx.x + H, ly = threadIdx.y + H;
+  const F* us = a.us;
+  const long long sz = a.sz;
+  const F ix = a.ix, iy = a.iy, iz = a.iz;
+
+  F zw[7];  // zw[s+3] = u(i, j, k+s)
+#pragma unroll
+  for (int s = 0; s < 6; s++) zw[s] = own_in ? us[own + sz * (kb - H + s - gb.klo)] : F(0);
+  F dz[5];  // dz[t] = D(k-1+t), D(m) = u(m) - u(m-1) along z
+#pragma unroll
+  for (int q = 0; q < 4; q++) dz[q] = zw[q + 2] - zw[q + 1];
+  Win wz = win5(zw[1] - zw[0], dz[0], dz[1], dz[2], dz[3]);  // window D(k-2..k+2) of the first plane
+
+  for (int k = kb; k <= ke; k++) {
+    const long long zoff = sz * (k - gb.klo);
+    zw[6] = own_in ? us[own + zoff + H * sz] : F(0);
+    const F hv = hoff >= 0 ? us[hoff + zoff] : F(0);
+    __syncthreads();
+    plane[ly * PX + lx] = zw[3];
+    if (tid < NHALO) plane[hslot] = hv;
+    __syncthreads();
+    // z: plus(k) from the window D(k-1..k+3), minus(k) from the carried D(k-2..k+2)
+    dz[4] = zw[6] - zw[5];
+    const Win wz1 = win5(dz[0], dz[1], dz[2], dz[3], dz[4]);
+    const float zp = rev(wz1, dz[1], dz[2], dz[3]) * iz;
+    const float zm = fwd(wz, dz[0], dz[1], dz[2]) * iz;
+    wz = wz1;
+    // x
+    F dx_[6], dy_[6];
+#pragma unroll
+    for (int q = -2; q <= 3; q++) {
+      dx_[q + 2] = plane[ly * PX + lx + q] - plane[ly * PX + lx + q - 1];
+      dy_[q + 2] = plane[(ly + q) * PX + lx] - plane[(ly + q - 1) * PX + lx];
+    }
+    const Win wx = win5(dx_[0], dx_[1], dx_[2], dx_[3], dx_[4]);
+    const float xm = fwd(wx, dx_[1], dx_[2], dx_[3]) * ix;
+    const float xp = rev(win5(dx_[1], dx_[2], dx_[3], dx_[4], dx_[5]), dx_[2], dx_[3], dx_[4]) * ix;
+    // y
+    const Win wy = win5(dy_[0], dy_[1], dy_[2], dy_[3], dy_[4]);
+    const float ym = fwd(wy, dy_[1], dy_[2], dy_[3]) * iy;
+    ywin[threadIdx.y][threadIdx.x] = wy;
+    __syncthreads();
+    const Win wyn = threadIdx.y == TY - 1 ? win5(dy_[1], dy_[2], dy_[3], dy_[4], dy_[5])
+                                          : ywin[threadIdx.y + 1][threadIdx.x];
+    const float yp = rev(wyn, dy_[2], dy_[3], dy_[4]) * iy;
+    if (active) {
+      const long long off = own + zoff;
+      F r;
+      if (RHS == kReinit) {
+        const F s0 = a.use_phi0 ? a.phi0[off] : zw[3];
+        r = reinit_rhs(s0, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
+      } else {
+        const F v = RHS == kNormalField ? a.vnf[off] : a.vn;
+        r = 0.f;
+        if (fabsf(v) >= kZeroTol)
+          r = r - v * __builtin_amdgcn_sqrtf(stencil::godunov_sq(v > 0.f, xp, yp, zp, xm, ym, zm));
+      }
+      const F uc = STAGE == 1 ? zw[3] : a.uc[off];
+      a.out[off] = rk_combine<STAGE>(uc, zw[3], a.dt, r);
+    }
+#pragma unroll
+    for (int s = 0; s < 6; s++) zw[s] = zw[s + 1];
+#pragma unroll
+    for (int q = 0; q < 4; q++) dz[q] = dz[q + 1];
+  }
+}
+
+template <int RHS, int STAGE>
+void launch_stage(const FusedArgs& a, int nblocks) {
+  if constexpr (sizeof(real_t) == 4 && (RHS == kNormalConst || RHS == kNormalField || RHS == kReinit))
+    hipLaunchKernelGGL((fused_pm_f32_kernel<RHS, STAGE, real_t>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+  else
+    hipLaunchKernelGGL((fused_stage_kernel<RHS, STAGE>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+}
+
 template <int RHS>
 bool launch_rhs(int stage, const FusedArgs& a, int nblocks) {
   switch (stage) {
-    case 1: hipLaunchKernelGGL((fused_stage_kernel<RHS, 1>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
-    case 2: hipLaunchKernelGGL((fused_stage_kernel<RHS, 2>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
-    case 3: hipLaunchKernelGGL((fused_stage_kernel<RHS, 3>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
-    case 22: hipLaunchKernelGGL((fused_stage_kernel<RHS, 22>), dim3(nblocks), dim3(TX, TY), 0, stream(), a); break;
+    case 1: launch_stage<RHS, 1>(a, nblocks); break;
+    case 2: launch_stage<RHS, 2>(a, nblocks); break;
+    case 3: launch_stage<RHS, 3>(a, nblocks); break;
+    case 22: launch_stage<RHS, 22>(a, nblocks); break;
     default: return false;
   }
   return true;

@@ -237,34 +237,65 @@ namespace fastf {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
-  // max(v_k^2) == (max |v_k|)^2 exactly (rounding is monotonic): two max3 and a multiply
-  const float ma = fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fabsf(v3)), fmaxf(fabsf(v4), fabsf(v5)));
-  const float im = rcp(fmaxf(ma * ma, 1e-30f));
+// One WENO5 stencil (v1..v5) in two parts.  win5 holds what depends on the
+// five values as a set -- the normalised smoothness products and the
+// candidate differences; fwd finishes weno5(v1, .., v5) and rev finishes
+// weno5(v5, .., v1), whose smoothness indicators are those of the forward
+// order taken in reverse.  The plus derivative at a cell and the minus
+// derivative at the next cell along an axis use the same five differences in
+// opposite orders, so one window serves both.
+//
+// The nonlinear weights are unchanged when the smoothness indicators S_k and
+// eps = 1e-6 max v^2 are scaled together: win5 divides them by max v^2
+// ((max |v|)^2, exact as rounding is monotonic) and multiplies by 4, so that
+// 4 S_k = (13/3) a_k^2 + b_k^2 with a_k, b_k the reference's two terms.  With
+// alpha_k = d_k / e_k^2 over the common denominator e1^2 e2^2 e3^2 and
+// p1 - p2 = (a1 - a2)/3, p3 - p2 = (a2 - a3)/6,
+//   sum_k w_k p_k = p2 + (0.1 q23 (a1 - a2)/3 + 0.3 q12 (a2 - a3)/6) / sum_k d_k q_(other two).
+struct Win {
+  float q12, q13, q23, d12, d23;
+};
+
+__device__ __forceinline__ Win win5(float v1, float v2, float v3, float v4, float v5) {
+  const float ma = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(v1), fabsf(v2)), fabsf(v3)), fabsf(v4)), fabsf(v5));
+  const float im = rcp(fmaf(ma, ma, 1e-30f));
   const float a1 = fmaf(-2.f, v2, v1) + v3, b1 = fmaf(3.f, v3, fmaf(-4.f, v2, v1));
   const float a2 = fmaf(-2.f, v3, v2) + v4, b2 = v2 - v4;
   const float a3 = fmaf(-2.f, v4, v3) + v5, b3 = fmaf(3.f, v3, fmaf(-4.f, v4, v5));
-  const float c13 = 13.f / 12.f;
-  const float s1 = fmaf(c13, a1 * a1, 0.25f * (b1 * b1));
-  const float s2 = fmaf(c13, a2 * a2, 0.25f * (b2 * b2));
-  const float s3 = fmaf(c13, a3 * a3, 0.25f * (b3 * b3));
-  const float e1 = fmaf(s1, im, 1e-6f), e2 = fmaf(s2, im, 1e-6f), e3 = fmaf(s3, im, 1e-6f);
+  const float e1 = fmaf(fmaf(a1 * (13.f / 3.f), a1, b1 * b1), im, 4e-6f);
+  const float e2 = fmaf(fmaf(a2 * (13.f / 3.f), a2, b2 * b2), im, 4e-6f);
+  const float e3 = fmaf(fmaf(a3 * (13.f / 3.f), a3, b3 * b3), im, 4e-6f);
   const float q1 = e1 * e1, q2 = e2 * e2, q3 = e3 * e3;
-  const float w1 = 0.1f * (q2 * q3), w2 = 0.6f * (q1 * q3), w3 = 0.3f * (q1 * q2);
-  const float p1 = fmaf(1.f / 3.f, v1, fmaf(-7.f / 6.f, v2, (11.f / 6.f) * v3));
+  return Win{q1 * q2, q1 * q3, q2 * q3, a1 - a2, a2 - a3};
+}
+// weno5(v1, .., v5) from win5(v1, .., v5)
+__device__ __forceinline__ float fwd(const Win& w, float v2, float v3, float v4) {
+  const float den = fmaf(0.1f, w.q23, fmaf(0.6f, w.q13, 0.3f * w.q12));
+  const float num = fmaf(w.q23, w.d12 * (0.1f / 3.f), w.q12 * (w.d23 * 0.05f));
   const float p2 = fmaf(-1.f / 6.f, v2, fmaf(5.f / 6.f, v3, (1.f / 3.f) * v4));
-  const float p3 = fmaf(1.f / 3.f, v3, fmaf(5.f / 6.f, v4, (-1.f / 6.f) * v5));
-  const float num = fmaf(w1, p1, fmaf(w2, p2, w3 * p3));
-  return num * rcp(w1 + w2 + w3);
+  return fmaf(num, rcp(den), p2);
+}
+// weno5(v5, .., v1) from win5(v1, .., v5): S1 and S3 trade places, and the
+// candidate differences become -(a2 - a3) and -(a1 - a2)
+__device__ __forceinline__ float rev(const Win& w, float v2, float v3, float v4) {
+  const float den = fmaf(0.1f, w.q12, fmaf(0.6f, w.q13, 0.3f * w.q23));
+  const float num = -fmaf(w.q12, w.d23 * (0.1f / 3.f), w.q23 * (w.d12 * 0.05f));
+  const float p2 = fmaf(-1.f / 6.f, v4, fmaf(5.f / 6.f, v3, (1.f / 3.f) * v2));
+  return fmaf(num, rcp(den), p2);
 }
 
-// Plus and minus WENO5 derivatives from seven samples f[s+3] along an axis.
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
+  return fwd(win5(v1, v2, v3, v4, v5), v2, v3, v4);
+}
+
+// Plus and minus WENO5 derivatives from seven samples f[s+3] along an axis:
+// minus uses D(-2..2), plus uses D(3..-1) = the window D(-1..3) reversed.
 __device__ __forceinline__ void plus_minus(const float f[7], float inv, float& p, float& m) {
   float d[6];
 #pragma unroll
   for (int s = -2; s <= 3; s++) d[s + 2] = f[s + 3] - f[s + 2];
-  p = weno5(d[5], d[4], d[3], d[2], d[1]) * inv;
-  m = weno5(d[0], d[1], d[2], d[3], d[4]) * inv;
+  p = rev(win5(d[1], d[2], d[3], d[4], d[5]), d[2], d[3], d[4]) * inv;
+  m = fwd(win5(d[0], d[1], d[2], d[3], d[4]), d[1], d[2], d[3]) * inv;
 }
 
 // Velocity-upwinded WENO5 derivative (vpos: backward/minus stencil).
@@ -275,6 +306,26 @@ __device__ __forceinline__ float upwind(const float f[7], float inv, bool vpos) 
   const float a = vpos ? d[0] : d[5], b = vpos ? d[1] : d[4], c = vpos ? d[2] : d[3];
   const float e = vpos ? d[3] : d[2], g = vpos ? d[4] : d[1];
   return weno5(a, b, c, e, g) * inv;
+}
+
+// Reinitialization RHS of one cell (the reference's formula,
+// lsm_reinitialization3d.f.in:167-199 and :251) with the hardware square root
+// and reciprocal square root (about 1 ulp each) in place of the correctly
+// rounded FP32 sqrt and division sequences.
+__device__ __forceinline__ float reinit_rhs(float s, bool use_phi0, float xp, float yp, float zp, float xm, float ym,
+                                            float zm, float dx_sq) {
+  const float gp[3] = {xp, yp, zp}, gm[3] = {xm, ym, zm};
+  float n2 = 0.f;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const float p = s > 0.f ? fmaxf(-gp[d], 0.f) : fmaxf(gp[d], 0.f);
+    const float m = s > 0.f ? fmaxf(gm[d], 0.f) : fmaxf(-gm[d], 0.f);
+    const float st = fmaxf(p, m);
+    n2 = fmaf(st, st, n2);
+  }
+  if (!(fabsf(s) >= kZeroTol)) return 0.f;
+  const float sgn = s * __builtin_amdgcn_rsqf(fmaf(s, s, use_phi0 ? dx_sq : n2 * dx_sq));
+  return sgn * (1.f - __builtin_amdgcn_sqrtf(n2));
 }
 
 }  // namespace fastf

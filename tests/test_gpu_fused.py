@@ -109,11 +109,13 @@ def test_fused_rk3_step_fp64_equals_dropin_chain(solvers, kind):
     assert np.array_equal(ha, hb), np.abs(ha - hb).max()
 
 
-@pytest.mark.parametrize("kind", ["normal", "advect"])
+@pytest.mark.parametrize("kind", ["normal", "advect", "reinit"])
 def test_fused_fp32_close_to_fp64(solvers, kind):
     g, s64, s32 = solvers
     from lsmlib_amd.solver import phi_sphere
     phi_np = phi_sphere(g, (0.5, 0.45, 0.42), 0.3)
+    if kind == "reinit":
+        phi_np = 1.7 * phi_np  # |grad phi| = 1.7: the reinitialization RHS is O(1)
     res = []
     for s in (s64, s32):
         a = s.field(phi_np)
@@ -122,6 +124,8 @@ def test_fused_fp32_close_to_fp64(solvers, kind):
         for step in range(5):
             if kind == "normal":
                 s.rk3_fused(a, w, s.stage_normal, 0.4 * g.dx[0], vn=1.0)
+            elif kind == "reinit":
+                s.rk3_fused(a, w, s.stage_reinit, 0.4 * g.dx[0])
             else:
                 s.rk3_fused(a, w, s.stage_deform, 0.4 * g.dx[0], c=1.0)
         s.lib.check()
