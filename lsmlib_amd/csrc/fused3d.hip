@@ -160,11 +160,30 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
     ta1 = a.ta[1][j - gb.jlo];
     tb1 = a.tb[1][j - gb.jlo];
   }
+  // FP32: the deformation velocity divided by the grid spacing, u/dx = X0 B(z),
+  // v/dy = X1 B(z), w/dz = X2 A(z), with the k-independent factors formed once
+  const real_t X0 = real_t(2) * a.tfac * a.ix * ta0 * tb1, X1 = -(a.tfac * a.iy) * tb0 * ta1,
+               X2 = -(a.tfac * a.iz) * tb0 * tb1;
 
   for (int k = kb; k <= ke; k++) {
     const long long zoff = sz * (k - gb.klo);
     zw[6] = own_in ? us[own + zoff + H * sz] : real_t(0);
     const real_t hv = hoff >= 0 ? us[hoff + zoff] : real_t(0);
+    // per-cell inputs of this plane, issued before the stencil work so that their latency is hidden
+    const long long off = own + zoff;
+    const real_t ucv = STAGE != 1 && active ? a.uc[off] : real_t(0);
+    real_t c0 = real_t(0), c1 = real_t(0), c2 = real_t(0);
+    if (active) {
+      if (RHS == kAdvectField) {
+        c0 = a.vx[off];
+        c1 = a.vy[off];
+        c2 = a.vz[off];
+      } else if (RHS == kReinit) {
+        c0 = a.use_phi0 ? a.phi0[off] : real_t(0);
+      } else if (RHS == kNormalField) {
+        c0 = a.vnf[off];
+      }
+    }
     __syncthreads();  // the previous plane's reads are done
     plane[ly * PX + lx] = zw[3];
     if (tid < NHALO) plane[hslot] = hv;
@@ -177,14 +196,21 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
         fy[s + 3] = plane[(ly + s) * PX + lx];
         fz[s + 3] = zw[s + 3];
       }
-      const long long off = own + zoff;
       real_t r;
-      if (RHS == kAdvectField || RHS == kAdvectDeform) {
+      if (sizeof(real_t) == 4 && RHS == kAdvectDeform) {
+        // r = -(u phi_x + v phi_y + w phi_z) with the derivatives' 1/dx folded into the velocity;
+        // |u| < zero_tol  <=>  |u/dx| < zero_tol/dx
+        const real_t ux = X0 * a.tb[2][k - gb.klo], vy = X1 * a.tb[2][k - gb.klo], wz = X2 * a.ta[2][k - gb.klo];
+        const real_t px = fabs(ux) < kZeroTol * a.ix ? real_t(0) : weno_up(fx, real_t(1), ux > real_t(0));
+        const real_t py = fabs(vy) < kZeroTol * a.iy ? real_t(0) : weno_up(fy, real_t(1), vy > real_t(0));
+        const real_t pz = fabs(wz) < kZeroTol * a.iz ? real_t(0) : weno_up(fz, real_t(1), wz > real_t(0));
+        r = real_t(0) - (ux * px + vy * py + wz * pz);
+      } else if (RHS == kAdvectField || RHS == kAdvectDeform) {
         real_t u, v, w;
         if (RHS == kAdvectField) {
-          u = a.vx[off];
-          v = a.vy[off];
-          w = a.vz[off];
+          u = c0;
+          v = c1;
+          w = c2;
         } else {
           const real_t ta2 = a.ta[2][k - gb.klo], tb2 = a.tb[2][k - gb.klo], c = a.tfac;
           u = real_t(2) * ta0 * tb1 * tb2 * c;
@@ -201,15 +227,15 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
         weno_pm(fy, a.iy, yp, ym);
         weno_pm(fz, a.iz, zp, zm);
         if (RHS == kReinit) {
-          const real_t s = a.use_phi0 ? a.phi0[off] : fx[3];
+          const real_t s = a.use_phi0 ? c0 : fx[3];
           r = stage_reinit(s, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
         } else {
-          const real_t v = RHS == kNormalField ? a.vnf[off] : a.vn;
+          const real_t v = RHS == kNormalField ? c0 : a.vn;
           r = real_t(0);
           if (fabs(v) >= kZeroTol) r = r - v * stage_sqrt(stencil::godunov_sq(v > real_t(0), xp, yp, zp, xm, ym, zm));
         }
       }
-      const real_t uc = STAGE == 1 ? fx[3] : a.uc[off];
+      const real_t uc = STAGE == 1 ? fx[3] : ucv;
       a.out[off] = rk_combine<STAGE>(uc, fx[3], a.dt, r);
     }
 #pragma unroll
@@ -220,12 +246,11 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
 // FP32 stages that need both one-sided derivatives (normal motion and
 // reinitialization).  The plus derivative at a cell and the minus derivative
 // at the next cell along an axis share one WENO5 window (stencil::fastf::win5),
-// so each window is formed once along z (the thread carries the window of
-// plane k+1 to the next plane) and along y (the windows of a tile row go
-// through LDS; the last row of the tile forms its neighbour's itself, one
-// wave in eight).  Along x both windows are formed in the lane: a neighbour's
-// window would come from the next lane, but the tile's last lane would then
-// form its own in a divergent branch, which costs the whole wave as much.
+// so each window is formed once: along z the thread carries the window of
+// plane k+1 to the next plane; along x and y the windows of the tile go
+// through LDS, and the windows one cell past the tile are formed by a single
+// wave (the last tile row for y, eight lanes of the first wave for x), so that
+// no wave runs a divergent branch for its edge lane.
 using stencil::fastf::Win;
 
 // F is real_t (float): a template parameter so that the FP64 build, which never
@@ -235,6 +260,7 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
   using namespace stencil::fastf;
   __shared__ F plane[PLANE];
   __shared__ Win ywin[TY][TX];
+  __shared__ Win xwin[TY][TX + 1];  // column TX: the windows one cell past the tile
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int nt = a.nti * a.ntj * a.ntk;
   const int b = blockIdx.x;
@@ -277,6 +303,13 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
     const long long zoff = sz * (k - gb.klo);
     zw[6] = own_in ? us[own + zoff + H * sz] : F(0);
     const F hv = hoff >= 0 ? us[hoff + zoff] : F(0);
+    const long long off = own + zoff;
+    const F ucv = STAGE != 1 && active ? a.uc[off] : F(0);
+    F c0 = F(0);
+    if (active) {
+      if (RHS == kReinit) c0 = a.use_phi0 ? a.phi0[off] : F(0);
+      else if (RHS == kNormalField) c0 = a.vnf[off];
+    }
     __syncthreads();
     plane[ly * PX + lx] = zw[3];
     if (tid < NHALO) plane[hslot] = hv;
@@ -296,7 +329,14 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
     }
     const Win wx = win5(dx_[0], dx_[1], dx_[2], dx_[3], dx_[4]);
     const float xm = fwd(wx, dx_[1], dx_[2], dx_[3]) * ix;
-    const float xp = rev(win5(dx_[1], dx_[2], dx_[3], dx_[4], dx_[5]), dx_[2], dx_[3], dx_[4]) * ix;
+    xwin[threadIdx.y][threadIdx.x] = wx;
+    if (threadIdx.y == 0 && threadIdx.x < TY) {  // one wave forms the TY windows past the tile's last column
+      const int row = (threadIdx.x + H) * PX + TX + H;
+      F e[5];
+#pragma unroll
+      for (int q = -2; q <= 2; q++) e[q + 2] = plane[row + q] - plane[row + q - 1];
+      xwin[threadIdx.x][TX] = win5(e[0], e[1], e[2], e[3], e[4]);
+    }
     // y
     const Win wy = win5(dy_[0], dy_[1], dy_[2], dy_[3], dy_[4]);
     const float ym = fwd(wy, dy_[1], dy_[2], dy_[3]) * iy;
@@ -305,19 +345,19 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
     const Win wyn = threadIdx.y == TY - 1 ? win5(dy_[1], dy_[2], dy_[3], dy_[4], dy_[5])
                                           : ywin[threadIdx.y + 1][threadIdx.x];
     const float yp = rev(wyn, dy_[2], dy_[3], dy_[4]) * iy;
+    const float xp = rev(xwin[threadIdx.y][threadIdx.x + 1], dx_[2], dx_[3], dx_[4]) * ix;
     if (active) {
-      const long long off = own + zoff;
       F r;
       if (RHS == kReinit) {
-        const F s0 = a.use_phi0 ? a.phi0[off] : zw[3];
+        const F s0 = a.use_phi0 ? c0 : zw[3];
         r = reinit_rhs(s0, a.use_phi0 != 0, xp, yp, zp, xm, ym, zm, a.dx_sq);
       } else {
-        const F v = RHS == kNormalField ? a.vnf[off] : a.vn;
+        const F v = RHS == kNormalField ? c0 : a.vn;
         r = 0.f;
         if (fabsf(v) >= kZeroTol)
           r = r - v * __builtin_amdgcn_sqrtf(stencil::godunov_sq(v > 0.f, xp, yp, zp, xm, ym, zm));
       }
-      const F uc = STAGE == 1 ? zw[3] : a.uc[off];
+      const F uc = STAGE == 1 ? zw[3] : ucv;
       a.out[off] = rk_combine<STAGE>(uc, zw[3], a.dt, r);
     }
 #pragma unroll
