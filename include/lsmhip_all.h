@@ -6,6 +6,7 @@
 #include "lsm_curvature3d.h"
 #include "lsm_curvature3d_local.h"
 #include "lsm_data_io.h"
+#include "lsm_field_extension3d.h"
 #include "lsm_geometry3d.h"
 #include "lsm_level_set_evolution3d.h"
 #include "lsm_level_set_evolution3d_local.h"

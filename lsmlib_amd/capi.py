@@ -56,6 +56,23 @@ _RAW = {
     "lsm3dcomputereinitializationeqnrhs_": "R:rhs B:rhs R:phi B:phi R:phi0 B:phi0 R:pxp R:pyp R:pzp B:gp R:pxm R:pym "
                                            "R:pzm B:gm B:fb S:dx S:dy S:dz I:use_phi0",
     "lsm3dcomputemeancurvatureorder2_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g B:fb S:dx S:dy S:dz",
+    # --- normals and curvatures, src/geometry/lsm_geometry3d.h, lsm_curvature3d.h
+    "lsm3dcomputeunitnormal_": "R:nx R:ny R:nz B:nrm R:px R:py R:pz B:g B:fb",
+    "lsm3dcomputesignedunitnormal_": "R:nx R:ny R:nz B:nrm R:px R:py R:pz B:g R:phi B:phi B:fb S:dx S:dy S:dz",
+    "lsm3dcomputemeancurvature_": "R:kappa B:kappa R:px R:py R:pz B:g R:pxx R:pxy R:pxz R:pyy R:pyz R:pzz B:g2 "
+                                  "B:fb",
+    "lsm3dcomputegaussiancurvatureorder2_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g B:fb "
+                                            "S:dx S:dy S:dz",
+    "lsm3dcomputemeancurvatureorder4_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g B:fb S:dx S:dy S:dz",
+    "lsm3dcomputegaussiancurvatureorder4_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g B:fb "
+                                            "S:dx S:dy S:dz",
+    # --- orthogonalization, src/reinitialization/lsm_reinitialization3d.h ------
+    "lsm3dcomputeorthogonalizationeqnrhs_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:psi "
+                                            "B:psi R:qxp R:qyp R:qzp B:qp R:qxm R:qym R:qzm B:qm B:fb "
+                                            "S:dx S:dy S:dz",
+    # --- field extension, src/field_extension/lsm_field_extension3d.h ----------
+    "lsm3dcomputefieldextensioneqnrhs_": "R:rhs B:rhs R:S B:S R:phi B:phi R:sx R:sy R:sz B:sg R:nx R:ny R:nz "
+                                         "B:nrm B:fb S:dx S:dy S:dz",
     # --- TVD Runge-Kutta, src/toolbox/lsm_tvd_runge_kutta3d.h -----------------
     "lsm3drk1step_": "R:un B:un R:uc B:uc R:rhs B:rhs B:fb S:dt",
     "lsm3dtvdrk2stage1_": "R:un B:un R:uc B:uc R:rhs B:rhs B:fb S:dt",

@@ -551,3 +551,154 @@ def test_periodic_fill_copies_opposite_interior():
                           indexing="ij")
     src = phi[(k % n[2]) + 3, (j % n[1]) + 3, (i % n[0]) + 3]
     assert np.array_equal(phi, src)
+
+
+# ---------------------------------------------------------------------------
+# Normals, curvatures, orthogonalization and field extension
+# ---------------------------------------------------------------------------
+def _sphere(n, d, r=0.3):
+    """phi = |x - c| - r on cells of spacing d (anisotropic), with its exact
+    gradient and Hessian; returns (gb, fb, phi, grad, hess, radius)."""
+    gb, fb, i, j, k = _grid((n, n, n))
+    x, y, z = (i + 0.5) * d[0] - 0.5, (j + 0.5) * d[1] - 0.5, (k + 0.5) * d[2] - 0.5
+    rr = np.sqrt(x * x + y * y + z * z)
+    X = (x, y, z)
+    grad = [a / rr for a in X]
+    hess = {}
+    for p, q in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)):
+        hess[p, q] = ((1.0 if p == q else 0.0) - X[p] * X[q] / rr ** 2) / rr
+    return gb, fb, rr - r, grad, hess, rr
+
+
+def _curvatures(n, d):
+    gb, fb, phi, (px, py, pz), _, rr = _sphere(n, d)
+    gm = np.ones(phi.shape)
+    out = {}
+    for name in ("meancurvatureorder4", "gaussiancurvatureorder4", "gaussiancurvatureorder2"):
+        kap = np.zeros(phi.shape)
+        getattr(O(), "lsm3dcompute" + name)(kap, gb, phi, gb, px, py, pz, gm, gb, fb, *d)
+        near = np.abs(_fb(phi, gb, fb)) < 2 * max(d)
+        want = (2 / _fb(rr, gb, fb) if name.startswith("mean") else _fb(rr, gb, fb) ** -2.0)[near]
+        out[name] = np.abs(_fb(kap, gb, fb)[near] / want - 1).max()
+    return out
+
+
+def test_curvature_orders_on_sphere_anisotropic():
+    """Fourth-order mean and Gaussian curvature and second-order Gaussian
+    curvature of a sphere with exact first derivatives on a grid with three
+    different spacings: the error against the level set through each cell
+    (2/r, 1/r^2) falls at rate >= 3.6 (order 4) and >= 1.8 (order 2) from
+    n = 32 to 64.  A spacing applied to the wrong axis, a wrong weight or a
+    mixed derivative taken along the wrong axis leaves an O(1) error."""
+    d = lambda n: (1.0 / n, 1.2 / n, 0.85 / n)  # noqa: E731
+    e1, e2 = _curvatures(32, d(32)), _curvatures(64, d(64))
+    for name, order in (("meancurvatureorder4", 3.6), ("gaussiancurvatureorder4", 3.6),
+                        ("gaussiancurvatureorder2", 1.8)):
+        rate = np.log2(e1[name] / e2[name])
+        assert e2[name] < 1e-2 and rate >= order, (name, e1[name], e2[name], rate)
+
+
+def test_mean_curvature_from_exact_hessian():
+    """lsm3dComputeMeanCurvature with the exact gradient and Hessian of a
+    sphere gives 2/r to rounding; with the gradient scaled by 7 (|grad|^3 in
+    the denominator, |grad|^2 in the numerator) it gives 2/(7 r)."""
+    gb, fb, phi, g, hs, rr = _sphere(20, (0.05, 0.05, 0.05))
+    o = O()
+    for scale in (1.0, 7.0):
+        kap = np.zeros(phi.shape)
+        o.lsm3dcomputemeancurvature(kap, gb, *[a * scale for a in g], gb, hs[0, 0], hs[0, 1], hs[0, 2], hs[1, 1],
+                                    hs[1, 2], hs[2, 2], gb, fb)
+        assert np.abs(_fb(kap * rr * scale, gb, fb) / 2 - 1).max() < 1e-12
+
+
+def test_unit_normals_closed_forms():
+    """Unit normal = grad/|grad| (and (1,0,0) where grad = 0); the signed unit
+    normal is sgn(phi) grad/|grad| with the smoothed sign
+    phi/sqrt(phi^2 + |grad|^2 max(d)^2): -> +-1 far from phi = 0, 0 at phi = 0."""
+    d = (0.05, 0.07, 0.04)
+    gb, fb, phi, g, _, _ = _sphere(20, d)
+    g = [a * 3.0 for a in g]  # |grad| = 3
+    g[0].reshape(-1)[::7] = 0.0
+    g[1].reshape(-1)[::7] = 0.0
+    g[2].reshape(-1)[::7] = 0.0
+    phi.reshape(-1)[3::11] = 0.0
+    o = O()
+    n = [np.zeros(phi.shape) for _ in range(3)]
+    o.lsm3dcomputeunitnormal(*n, gb, *g, gb, fb)
+    zero = _fb(g[0] == 0, gb, fb)
+    norm = np.sqrt(sum(_fb(a, gb, fb) ** 2 for a in g))
+    for a, ga, e in zip(n, g, (1.0, 0.0, 0.0)):
+        assert np.all(_fb(a, gb, fb)[zero] == e)
+        assert np.abs(_fb(a, gb, fb)[~zero] - _fb(ga, gb, fb)[~zero] / norm[~zero]).max() < 1e-15
+    o.lsm3dcomputesignedunitnormal(*n, gb, *g, gb, phi, gb, fb, *d)
+    P = _fb(phi, gb, fb)
+    at_zero = P == 0
+    A = [_fb(a, gb, fb) for a in n]
+    for a in A:
+        assert np.all(a[at_zero] == 0)
+    ok = ~zero & ~at_zero
+    dot = sum(a * _fb(ga, gb, fb) for a, ga in zip(A, g)) / 3.0  # n . grad/|grad|
+    length = np.sqrt(sum(a * a for a in A))
+    assert np.all(np.sign(dot[ok]) == np.sign(P[ok]))      # points along sgn(phi) grad phi
+    assert np.abs(length[ok] - np.abs(dot[ok])).max() < 1e-15  # and along grad phi only
+    assert np.all(np.abs(dot[ok]) < 1) and np.all(np.abs(dot[ok & (np.abs(P) > 0.3)]) > 0.8)
+    s = P / np.sqrt(P * P + 9.0 * max(d) ** 2)  # smoothed sign at |grad| = 3
+    assert np.abs(dot[ok] - s[ok]).max() < 1e-14
+
+
+def test_orthogonalization_rhs_linear_fields():
+    """For linear phi and psi (one-sided derivatives equal), the RHS is
+    -sgn(psi) (grad psi/|grad psi|) . grad phi: 0 for orthogonal gradients;
+    with different plus/minus derivatives the upwind side follows the sign of
+    sgn(psi) grad psi per component."""
+    d = (0.1, 0.12, 0.09)
+    gb, fb, i, j, k = _grid((8, 7, 6))
+    rng = np.random.default_rng(4)
+    o = O()
+    for trial in range(4):
+        a, b = rng.standard_normal(3), rng.standard_normal(3)
+        if trial == 0:
+            b = b - a * (a @ b) / (a @ a)  # orthogonal
+        psi = 0.2 + b[0] * i * d[0] + b[1] * j * d[1] + b[2] * k * d[2]
+        P = [np.full(psi.shape, v) for v in a]
+        Q = [np.full(psi.shape, v) for v in b]
+        rhs = np.zeros(psi.shape)
+        o.lsm3dcomputeorthogonalizationeqnrhs(rhs, gb, *P, gb, *P, gb, psi, gb, *Q, gb, *Q, gb, fb, *d)
+        S = _fb(psi, gb, fb)
+        s = S / np.sqrt(S * S + max(d) ** 2 * (b @ b))
+        want = -s * (a @ b) / np.sqrt(b @ b)
+        assert np.abs(_fb(rhs, gb, fb) - want).max() < 1e-12 * max(1.0, np.abs(want).max())
+    # upwinding: grad psi = (1,0,0); phi_x_minus = 1, phi_x_plus = 100
+    psi = np.where(i < 4, -0.5, 0.5)
+    one, zero = np.ones(psi.shape), np.zeros(psi.shape)
+    rhs = np.zeros(psi.shape)
+    o.lsm3dcomputeorthogonalizationeqnrhs(rhs, gb, 100 * one, zero, zero, gb, one, zero, zero, gb, psi, gb,
+                                          one, zero, zero, gb, one, zero, zero, gb, fb, *d)
+    S = _fb(psi, gb, fb)
+    s = S / np.sqrt(S * S + max(d) ** 2)
+    assert np.allclose(_fb(rhs, gb, fb), np.where(S > 0, -s * 1.0, -s * 100.0), rtol=1e-14, atol=0)
+
+
+def test_field_extension_rhs_on_sphere():
+    """S = r (distance from the centre), N the exact signed normal of the
+    sphere phi = r - R: the RHS is -N . grad S = -sign(phi) wherever phi has
+    the sign of its six face neighbours or |phi| > 3 max(d), and exactly 0 in
+    the cells the zero level set crosses."""
+    d = (0.05, 0.06, 0.045)
+    gb, fb, phi, g, _, rr = _sphere(20, d)
+    sg = np.sign(phi)
+    N = [sg * a for a in g]
+    rhs = np.zeros(phi.shape)
+    O().lsm3dcomputefieldextensioneqnrhs(rhs, gb, rr, gb, phi, gb, *g, gb, *N, gb, fb, *d)
+    P, R_ = _fb(phi, gb, fb), _fb(rhs, gb, fb)
+    lo = [f - b for f, b in zip(fb[::2], gb[::2])]
+    same = np.ones(P.shape, bool)
+    for ax in range(3):
+        for s in (-1, 1):
+            sl = [slice(lo[2 - a], lo[2 - a] + P.shape[a]) for a in range(3)]
+            sl[2 - ax] = slice(lo[ax] + s, lo[ax] + s + P.shape[2 - ax])
+            same &= P * phi[tuple(sl)] > 0
+    away = (np.abs(P) > 3 * max(d)) | same
+    assert np.all(R_[~away] == 0)
+    assert np.abs(R_[away] + np.sign(P[away])).max() < 1e-14
+    assert (~away).sum() > 100 and away.sum() > 1000

@@ -25,6 +25,15 @@ WHOLE_GRID = [
     "lsm3dvolumeregionphilessthanzero_", "lsm3dvolumeregionphigreaterthanzero_",
     "lsm3dsurfaceareazerolevelset_",
 ]
+# Kernels either side of the curvature / reinitialization path: normals,
+# curvatures, orthogonalization and field extension.
+NEIGHBOURS = [
+    "lsm3dcomputeunitnormal_", "lsm3dcomputesignedunitnormal_", "lsm3dcomputemeancurvature_",
+    "lsm3dcomputegaussiancurvatureorder2_", "lsm3dcomputemeancurvatureorder4_",
+    "lsm3dcomputegaussiancurvatureorder4_", "lsm3dcomputeorthogonalizationeqnrhs_",
+    "lsm3dcomputefieldextensioneqnrhs_",
+]
+ORDER4 = {"lsm3dcomputemeancurvatureorder4_", "lsm3dcomputegaussiancurvatureorder4_"}
 QUIRKY_BOUNDS = ["lsm3daddconstprecomputedcurvtermtolserhs_", "lsm3daddexternalandnormalveltermtolserhs_"]
 BCS = ["lsm3dlinearextrapolation_", "lsm3dsignedlinearextrapolation_", "lsm3dcopyextrapolation_",
        "lsm3dhomogeneousneumanneno1_", "lsm3dhomogeneousneumanneno2_", "lsm3dhomogeneousneumanneno3_",
@@ -44,6 +53,37 @@ def _prepare(case, sym):
         g.reshape(-1)[::17] = 0.0
     if sym.startswith("lsm3dcomputestableconstnormalveldt"):
         case.scalars["vn"] = -0.75
+
+
+def prepare_neighbours(case, sym):
+    """Inputs for NEIGHBOURS that take every branch: gradients exactly zero in
+    some cells and of size 1e-6 in others (|grad|^2 below the 1e-11 zero
+    tolerance while |grad| is above it), psi with zeros and tiny values, and
+    a gradient box two cells wider than the fill box for the fourth-order
+    curvatures, whose mixed derivatives read the gradient two cells away."""
+    if sym in ORDER4:
+        case.boxes["g"] = case.gb
+    boxes = H.box_of_array(sym)
+    names = [n for k, n in capi.SPECS[sym] if k == "R"]
+    for group in (("px", "py", "pz"), ("qxp", "qyp", "qzp"), ("qxm", "qym", "qzm")):
+        if group[0] in names:
+            for n in group:
+                f = case.array(n, boxes[n]).reshape(-1)
+                f[::13] = 0.0
+                f[5::29] *= 1e-6
+    if "psi" in names:
+        f = case.array("psi", boxes["psi"]).reshape(-1)
+        f[3::11] = 0.0
+        f[7::23] *= 1e-12
+    _prepare(case, sym)
+
+
+@pytest.mark.parametrize("sym", NEIGHBOURS)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_neighbour_kernels_bitwise(sym, seed):
+    c = H.grid_case(seed)
+    prepare_neighbours(c, sym)
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
 
 
 @pytest.mark.parametrize("sym", WHOLE_GRID)
