@@ -15,6 +15,8 @@ compared with the FP64 one (L-inf / L1 of phi and relative interface volume,
 FP64 Kahan-compensated).  Inputs are resident in HBM when timing starts.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-compare]
+    python bench.py --kernels [--n 512] [--steps K]   # normals, curvatures, orthogonalization,
+                                                      # field extension: HBM roofline per entry point
 """
 from __future__ import annotations
 
@@ -136,6 +138,116 @@ def cpu_baseline(n=80, steps=2):
                 n, steps, el)}
 
 
+# Entry points either side of the curvature / reinitialization path, timed
+# one by one at n^3: (C-ABI name, real arrays read per cell, real arrays
+# written per cell).  Algorithmic bytes = (read + written) x cells x scalar size.
+NEIGHBOUR_KERNELS = [
+    ("lsm3dcomputeunitnormal", 3, 3),
+    ("lsm3dcomputesignedunitnormal", 4, 3),
+    ("lsm3dcomputemeancurvature", 9, 1),
+    ("lsm3dcomputegaussiancurvatureorder2", 5, 1),
+    ("lsm3dcomputemeancurvatureorder4", 5, 1),
+    ("lsm3dcomputegaussiancurvatureorder4", 5, 1),
+    ("lsm3dcomputeorthogonalizationeqnrhs", 13, 1),
+    ("lsm3dcomputefieldextensioneqnrhs", 7, 1),
+]
+
+
+def kernel_table(n, reps, warmup):
+    """Average launch time (HIP events on the launch stream, `reps` launches
+    back to back) and HBM roofline fraction of each NEIGHBOUR_KERNELS entry
+    point at n^3 (3 ghost cells), FP64 and FP32, on a sphere of radius 0.3:
+    phi its signed distance, exact gradient, |grad phi| = 1, the Hessian of
+    the distance, psi a plane through the centre, S a smooth field."""
+    import torch
+    import lsmlib_amd
+    rows = []
+    # the practical bandwidth reference: one device copy of 1 GiB (reads 1 GiB, writes 1 GiB)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(warmup):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = 2 * src.numel() * 4 / (ms * 1e-3) / 1e9
+    rows.append({"kernel": "device copy 1 GiB (torch)", "dtype": "f32", "avg_launch_ms": round(ms, 4),
+                 "alg_bytes": 2 * src.numel() * 4, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    print("[bench] %-38s %s %8.4f ms %7.1f GB/s %5.1f%% of HBM peak" % (
+        "device copy 1 GiB", "f32", ms, gbs, 100 * gbs / HBM_PEAK_GBS), file=sys.stderr, flush=True)
+    del src, dst
+    g = 3
+    gb = (-g, n - 1 + g) * 3
+    fb = (0, n - 1) * 3
+    h = 1.0 / n
+    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        lib = lsmlib_amd.load(prec)
+        stream = torch.cuda.current_stream()
+        lib.set_stream(stream.cuda_stream, False)
+        ax = (torch.arange(-g, n + g, dtype=dt, device="cuda") + 0.5) * h - 0.5
+        z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+        r = torch.sqrt(x * x + y * y + z * z)
+        A = {"phi": r - 0.3, "px": x / r, "py": y / r, "pz": z / r}
+        del z, y, x
+        A["gmag"] = torch.ones_like(r)
+        for key, (p, q) in {"xx": (0, 0), "xy": (0, 1), "xz": (0, 2), "yy": (1, 1), "yz": (1, 2),
+                            "zz": (2, 2)}.items():
+            d = (A["px"], A["py"], A["pz"])
+            A["p" + key] = ((1.0 if p == q else 0.0) - d[p] * d[q]) / r
+        A["psi"] = A["py"] * r
+        A["S"] = torch.sin(4 * A["phi"]) + A["px"]
+        A["w"] = A["S"] * A["pz"]
+        del r
+        out = [torch.zeros_like(A["phi"]) for _ in range(3)]
+        P = A
+        calls = {
+            "lsm3dcomputeunitnormal": (*out, gb, P["px"], P["py"], P["pz"], gb, fb),
+            "lsm3dcomputesignedunitnormal": (*out, gb, P["px"], P["py"], P["pz"], gb, P["phi"], gb, fb, h, h, h),
+            "lsm3dcomputemeancurvature": (out[0], gb, P["px"], P["py"], P["pz"], gb, P["pxx"], P["pxy"], P["pxz"],
+                                          P["pyy"], P["pyz"], P["pzz"], gb, fb),
+            "lsm3dcomputegaussiancurvatureorder2": (out[0], gb, P["phi"], gb, P["px"], P["py"], P["pz"], P["gmag"],
+                                                    gb, fb, h, h, h),
+            "lsm3dcomputemeancurvatureorder4": (out[0], gb, P["phi"], gb, P["px"], P["py"], P["pz"], P["gmag"], gb,
+                                                fb, h, h, h),
+            "lsm3dcomputegaussiancurvatureorder4": (out[0], gb, P["phi"], gb, P["px"], P["py"], P["pz"], P["gmag"],
+                                                    gb, fb, h, h, h),
+            # thirteen distinct input arrays: grad phi plus/minus and grad psi plus/minus stand in
+            "lsm3dcomputeorthogonalizationeqnrhs": (out[0], gb, P["px"], P["py"], P["pz"], gb, P["pxx"], P["pyy"],
+                                                    P["pzz"], gb, P["psi"], gb, P["pxy"], P["pxz"], P["pyz"], gb,
+                                                    P["S"], P["gmag"], P["w"], gb, fb, h, h, h),
+            "lsm3dcomputefieldextensioneqnrhs": (out[0], gb, P["S"], gb, P["phi"], gb, P["pxx"], P["pyy"], P["pzz"],
+                                                 gb, P["px"], P["py"], P["pz"], gb, fb, h, h, h),
+        }
+        cells = n ** 3
+        size = 8 if prec == "f64" else 4
+        for name, rd, wr in NEIGHBOUR_KERNELS:
+            fn, args = getattr(lib.api, name), calls[name]
+            for _ in range(warmup):
+                fn(*args)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(*args)
+            e1.record(stream)
+            e1.synchronize()
+            lib.check()
+            ms = e0.elapsed_time(e1) / reps
+            alg = (rd + wr) * cells * size
+            gbs = alg / (ms * 1e-3) / 1e9
+            rows.append({"kernel": name, "dtype": prec, "avg_launch_ms": round(ms, 4), "alg_bytes": alg,
+                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            print("[bench] %-38s %s %8.4f ms %7.1f GB/s %5.1f%% of HBM peak" % (
+                name, prec, ms, gbs, 100 * gbs / HBM_PEAK_GBS), file=sys.stderr, flush=True)
+        lib.set_stream(0, True)
+        del A, P, out, calls
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +258,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 timings and the accuracy check")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=64)
+    ap.add_argument("--kernels", action="store_true",
+                    help="time the normal / curvature / orthogonalization / field-extension entry points one by "
+                         "one at n^3 (--steps launches each) instead of the level-set step")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device copy of 1 GiB before timing (the known-size kernel for HBM counters)")
     a = ap.parse_args()
@@ -157,6 +272,12 @@ def main():
     import torch
     from lsmlib_amd.solver import DeformationRun
 
+    if a.kernels:
+        rows = kernel_table(a.n, a.steps, a.warmup)
+        print(json.dumps({"metric": "drop-in entry point HBM roofline @%d^3" % a.n, "unit": "GB/s",
+                          "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                          "data": "synthetic", "kernels": rows}))
+        return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

@@ -6,9 +6,13 @@
 // References: src/geometry/lsm_geometry3d_fort.f.in, src/geometry/lsm_curvature3d.f.in,
 // src/reinitialization/lsm_reinitialization3d.f.in, src/field_extension/lsm_field_extension3d.f.
 //
-// One thread per cell of the fill box; every kernel reads each input once per
-// cell (stencil neighbours come from L1/L2), so all of them are bound by HBM
-// bandwidth.  In FP64 the operation order is the reference's, without
+// All of them are bound by HBM bandwidth.  A block covers a 64 x 4 tile of
+// (i, j) columns and each thread KC consecutive cells along k, tiles dealt to
+// the 8 XCDs in contiguous ranges (so in-plane neighbours of a tile are read
+// through the same L2).  Pointwise kernels load the inputs of all KC cells
+// before computing any (memory-level parallelism); stencil kernels march along
+// k carrying the k-neighbours in registers, so each input plane is read from
+// HBM once.  In FP64 the operation order is the reference's, without
 // contraction, and every result is bitwise the reference's except the mean
 // curvature's |grad phi|^3 = (|grad phi|^2)^1.5 (see pow_three_halves).
 #include "lsmhip_core.hpp"
@@ -57,8 +61,60 @@ __device__ __forceinline__ real_t pow_three_halves(real_t x) {
 }
 
 // ---------------------------------------------------------------------------
-// Normals, lsm_geometry3d_fort.f.in:27-272
+// Tiling
 // ---------------------------------------------------------------------------
+struct Tiles {
+  Box fb;
+  int nti, ntj, ntk, kc, per;
+};
+
+Tiles tiles(const Box& fb, int kc) {
+  Tiles t{fb, (fb.nx() + kBX - 1) / kBX, (fb.ny() + kBY - 1) / kBY, (fb.nz() + kc - 1) / kc, kc, 0};
+  t.per = (int)(((long long)t.nti * t.ntj * t.ntk + 7) / 8);
+  return t;
+}
+dim3 tile_grid(const Tiles& t) { return dim3((unsigned)(8 * t.per)); }
+
+// The column of one thread: cells (i, j, k0 .. k0 + n - 1).  Block b runs on
+// XCD b % 8; XCD x takes the tile range [x per, (x + 1) per).
+struct Column {
+  int i, j, k0, n;
+};
+
+__device__ __forceinline__ bool column(const Tiles& T, Column& c) {
+  const int b = blockIdx.x;
+  const int t = (b % 8) * T.per + b / 8;
+  if (t >= T.nti * T.ntj * T.ntk) return false;
+  const int ti = t % T.nti, tj = (t / T.nti) % T.ntj, tk = t / (T.nti * T.ntj);
+  c.i = T.fb.ilo + ti * kBX + (int)threadIdx.x;
+  c.j = T.fb.jlo + tj * kBY + (int)threadIdx.y;
+  c.k0 = T.fb.klo + tk * T.kc;
+  c.n = min(T.kc, T.fb.khi - c.k0 + 1);
+  return c.i <= T.fb.ihi && c.j <= T.fb.jhi;
+}
+
+// Pointwise kernels: Op::load(i, j, k) reads one cell's inputs, Op::store
+// computes and writes its outputs.  A full column issues all KC loads first.
+template <int KC, class Op>
+__device__ __forceinline__ void pointwise(const Column& c, const Op& op) {
+  if (c.n == KC) {
+    typename Op::In v[KC];
+#pragma unroll
+    for (int t = 0; t < KC; ++t) v[t] = op.load(c.i, c.j, c.k0 + t);
+#pragma unroll
+    for (int t = 0; t < KC; ++t) op.store(c.i, c.j, c.k0 + t, v[t]);
+  } else {
+    for (int t = 0; t < c.n; ++t) op.store(c.i, c.j, c.k0 + t, op.load(c.i, c.j, c.k0 + t));
+  }
+}
+
+template <int KC, class Op>
+__global__ void __launch_bounds__(256) pointwise_kernel(Tiles T, Op op) {
+  Column c;
+  if (!column(T, c)) return;
+  pointwise<KC>(c, op);
+}
+
 struct Vec3Out {
   View<real_t> x, y, z;
 };
@@ -66,71 +122,153 @@ struct Vec3In {
   View<const real_t> x, y, z;
 };
 
-__global__ void __launch_bounds__(256) unit_normal_kernel(Box fb, Vec3Out n, Vec3In g) {
-  LSMHIP_CELL(fb);
-  const real_t x = g.x(i, j, k), y = g.y(i, j, k), z = g.z(i, j, k);
-  const real_t norm = sqrt(x * x + y * y + z * z);
-  real_t ox = real_t(1), oy = real_t(0), oz = real_t(0);
-  if (norm >= kZeroTol) {
-    const real_t inv = real_t(1) / norm;
-    ox = x * inv;
-    oy = y * inv;
-    oz = z * inv;
-  }
-  n.x(i, j, k) = ox;
-  n.y(i, j, k) = oy;
-  n.z(i, j, k) = oz;
-}
-
-__global__ void __launch_bounds__(256) signed_unit_normal_kernel(Box fb, Vec3Out n, Vec3In g,
-                                                                 View<const real_t> phi, real_t dx_sq) {
-  LSMHIP_CELL(fb);
-  const real_t c = phi(i, j, k);
-  real_t ox = real_t(0), oy = real_t(0), oz = real_t(0);
-  if (fabs(c) > kZeroTol) {
-    const real_t x = g.x(i, j, k), y = g.y(i, j, k), z = g.z(i, j, k);
-    const real_t ngs = x * x + y * y + z * z;
-    if (ngs >= kZeroTol) {
-      const real_t sgn = c / sqrt(c * c + ngs * dx_sq);
-      const real_t inv = real_t(1) / sqrt(ngs);
-      ox = sgn * x * inv;
-      oy = sgn * y * inv;
-      oz = sgn * z * inv;
-    } else {
-      ox = real_t(1);
+// ---------------------------------------------------------------------------
+// Normals, lsm_geometry3d_fort.f.in:27-272
+// ---------------------------------------------------------------------------
+struct UnitNormal {
+  Vec3Out n;
+  Vec3In g;
+  struct In {
+    real_t x, y, z;
+  };
+  __device__ __forceinline__ In load(int i, int j, int k) const { return In{g.x(i, j, k), g.y(i, j, k), g.z(i, j, k)}; }
+  __device__ __forceinline__ void store(int i, int j, int k, const In& v) const {
+    const real_t norm = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    real_t ox = real_t(1), oy = real_t(0), oz = real_t(0);
+    if (norm >= kZeroTol) {
+      const real_t inv = real_t(1) / norm;
+      ox = v.x * inv;
+      oy = v.y * inv;
+      oz = v.z * inv;
     }
+    n.x(i, j, k) = ox;
+    n.y(i, j, k) = oy;
+    n.z(i, j, k) = oz;
   }
-  n.x(i, j, k) = ox;
-  n.y(i, j, k) = oy;
-  n.z(i, j, k) = oz;
-}
-
-// ---------------------------------------------------------------------------
-// Curvatures
-// ---------------------------------------------------------------------------
-struct Hessian {
-  View<const real_t> xx, xy, xz, yy, yz, zz;
 };
 
-// lsm_geometry3d_fort.f.in:274-394 -- numerator terms in that routine's order.
-__global__ void __launch_bounds__(256) mean_curvature_hessian_kernel(Box fb, View<real_t> kappa, Vec3In g,
-                                                                     Hessian h) {
-  LSMHIP_CELL(fb);
-  const real_t x = g.x(i, j, k), y = g.y(i, j, k), z = g.z(i, j, k);
-  const real_t ngs = x * x + y * y + z * z;
-  real_t out = real_t(0);
-  if (!(ngs < kZeroTol)) {
-    const real_t xx = h.xx(i, j, k), xy = h.xy(i, j, k), xz = h.xz(i, j, k), yy = h.yy(i, j, k),
-                 yz = h.yz(i, j, k), zz = h.zz(i, j, k);
-    const real_t num = xx * y * y - real_t(2) * xy * x * y + yy * x * x + xx * z * z - real_t(2) * xz * x * z +
-                       zz * x * x + yy * z * z - real_t(2) * yz * y * z + zz * y * y;
-    out = num / pow_three_halves(ngs);
+struct SignedUnitNormal {
+  Vec3Out n;
+  Vec3In g;
+  View<const real_t> phi;
+  real_t dx_sq;
+  struct In {
+    real_t c, x, y, z;
+  };
+  __device__ __forceinline__ In load(int i, int j, int k) const {
+    return In{phi(i, j, k), g.x(i, j, k), g.y(i, j, k), g.z(i, j, k)};
   }
-  kappa(i, j, k) = out;
+  __device__ __forceinline__ void store(int i, int j, int k, const In& v) const {
+    real_t ox = real_t(0), oy = real_t(0), oz = real_t(0);
+    if (fabs(v.c) > kZeroTol) {
+      const real_t ngs = v.x * v.x + v.y * v.y + v.z * v.z;
+      if (ngs >= kZeroTol) {
+        const real_t sgn = v.c / sqrt(v.c * v.c + ngs * dx_sq);
+        const real_t inv = real_t(1) / sqrt(ngs);
+        ox = sgn * v.x * inv;
+        oy = sgn * v.y * inv;
+        oz = sgn * v.z * inv;
+      } else {
+        ox = real_t(1);
+      }
+    }
+    n.x(i, j, k) = ox;
+    n.y(i, j, k) = oy;
+    n.z(i, j, k) = oz;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Mean curvature from a given Hessian, lsm_geometry3d_fort.f.in:274-394
+// (numerator terms in that routine's order)
+// ---------------------------------------------------------------------------
+struct MeanCurvatureHessian {
+  View<real_t> kappa;
+  Vec3In g;
+  View<const real_t> hxx, hxy, hxz, hyy, hyz, hzz;
+  struct In {
+    real_t x, y, z, xx, xy, xz, yy, yz, zz;
+  };
+  __device__ __forceinline__ In load(int i, int j, int k) const {
+    return In{g.x(i, j, k),   g.y(i, j, k),   g.z(i, j, k),   hxx(i, j, k), hxy(i, j, k),
+              hxz(i, j, k),   hyy(i, j, k),   hyz(i, j, k),   hzz(i, j, k)};
+  }
+  __device__ __forceinline__ void store(int i, int j, int k, const In& v) const {
+    const real_t x = v.x, y = v.y, z = v.z;
+    const real_t ngs = x * x + y * y + z * z;
+    real_t out = real_t(0);
+    if (!(ngs < kZeroTol)) {
+      const real_t num = v.xx * y * y - real_t(2) * v.xy * x * y + v.yy * x * x + v.xx * z * z -
+                         real_t(2) * v.xz * x * z + v.zz * x * x + v.yy * z * z - real_t(2) * v.yz * y * z +
+                         v.zz * y * y;
+      out = num / pow_three_halves(ngs);
+    }
+    kappa(i, j, k) = out;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Orthogonalization RHS, lsm_reinitialization3d.f.in:296-476
+// ---------------------------------------------------------------------------
+struct Orthogonalization {
+  View<real_t> rhs;
+  Vec3In phi_p, phi_m;
+  View<const real_t> psi;
+  Vec3In psi_p, psi_m;
+  real_t dx_sq;
+  struct In {
+    real_t s, qxp, qyp, qzp, qxm, qym, qzm, pxp, pyp, pzp, pxm, pym, pzm;
+  };
+  __device__ __forceinline__ In load(int i, int j, int k) const {
+    return In{psi(i, j, k),     psi_p.x(i, j, k), psi_p.y(i, j, k), psi_p.z(i, j, k), psi_m.x(i, j, k),
+              psi_m.y(i, j, k), psi_m.z(i, j, k), phi_p.x(i, j, k), phi_p.y(i, j, k), phi_p.z(i, j, k),
+              phi_m.x(i, j, k), phi_m.y(i, j, k), phi_m.z(i, j, k)};
+  }
+  __device__ __forceinline__ void store(int i, int j, int k, const In& v) const {
+    const real_t g1 = real_t(0.5) * (v.qxp + v.qxm);
+    const real_t g2 = real_t(0.5) * (v.qyp + v.qym);
+    const real_t g3 = real_t(0.5) * (v.qzp + v.qzm);
+    const real_t ng = sqrt(g1 * g1 + g2 * g2 + g3 * g3);
+    real_t out = real_t(0);
+    if (fabs(v.s) > kZeroTol && ng > kZeroTol) {
+      const real_t sgn = v.s / sqrt(v.s * v.s + dx_sq * (ng * ng));
+      const real_t a1 = sgn * g1, a2 = sgn * g2, a3 = sgn * g3;
+      const real_t z = real_t(0);
+      out = real_t(-1) / ng *
+            (maxr(a1, z) * v.pxm + minr(a1, z) * v.pxp + maxr(a2, z) * v.pym + minr(a2, z) * v.pyp +
+             maxr(a3, z) * v.pzm + minr(a3, z) * v.pzp);
+    }
+    rhs(i, j, k) = out;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Stencil kernels: march along k with the k-neighbours in registers.
+// Pointers are restrict-qualified (the reference's Fortran routines may
+// assume that no output aliases an input), so loads of later cells may be
+// issued ahead of earlier stores.
+// ---------------------------------------------------------------------------
+struct Lay {
+  int ilo, jlo, klo, sy, sz;
+  __device__ __forceinline__ int at(int i, int j, int k) const {
+    return (i - ilo) + sy * (j - jlo) + sz * (k - klo);
+  }
+};
+Lay lay(const Box& b) { return Lay{b.ilo, b.jlo, b.klo, b.nx(), b.nx() * b.ny()}; }
+
+// Runs step(t) for t = 0 .. n-1, fully unrolled when the column is full.
+template <int KC, class F>
+__device__ __forceinline__ void march(int n, F&& step) {
+  if (n == KC) {
+#pragma unroll
+    for (int t = 0; t < KC; ++t) step(t);
+  } else {
+    for (int t = 0; t < n; ++t) step(t);
+  }
 }
 
-// Gaussian numerator, lsm_curvature3d.f.in:230-237 (and :490-497 with the x-z
-// mixed derivative taken from phi_z along x).
+// Gaussian-curvature numerator, lsm_curvature3d.f.in:230-237 (and :490-497
+// with the x-z mixed derivative taken from phi_z along x).
 __device__ __forceinline__ real_t gauss_num(real_t x, real_t y, real_t z, real_t xx, real_t yy, real_t zz, real_t xy,
                                             real_t xz, real_t yz) {
   return x * x * (yy * zz - yz * yz) + y * y * (xx * zz - xz * xz) + z * z * (xx * yy - xy * xy) +
@@ -141,23 +279,46 @@ struct Factors2 {
   real_t xx, yy, zz, xy, xz, yz;
 };
 
+// phi at a cell and its four in-plane face neighbours.
+struct Cross {
+  real_t c, xm, xp, ym, yp;
+};
+__device__ __forceinline__ Cross cross_at(const real_t* __restrict__ F, int o, int sy) {
+  return Cross{F[o], F[o - 1], F[o + 1], F[o - sy], F[o + sy]};
+}
+
 // lsm_curvature3d.f.in:136-248: second differences of phi (2-point cross
-// stencil for the mixed terms), numerator over |grad phi|^4.
-__global__ void __launch_bounds__(256) gaussian_order2_kernel(Box fb, View<real_t> kappa, View<const real_t> F,
-                                                              Vec3In g, View<const real_t> gmag, Factors2 f) {
-  LSMHIP_CELL(fb);
-  const real_t c = F(i, j, k);
-  const real_t xx = (F(i + 1, j, k) - real_t(2) * c + F(i - 1, j, k)) * f.xx;
-  const real_t yy = (F(i, j + 1, k) - real_t(2) * c + F(i, j - 1, k)) * f.yy;
-  const real_t zz = (F(i, j, k + 1) - real_t(2) * c + F(i, j, k - 1)) * f.zz;
-  const real_t xy = (F(i + 1, j + 1, k) - F(i + 1, j - 1, k) - F(i - 1, j + 1, k) + F(i - 1, j - 1, k)) * f.xy;
-  const real_t xz = (F(i + 1, j, k + 1) - F(i + 1, j, k - 1) - F(i - 1, j, k + 1) + F(i - 1, j, k - 1)) * f.xz;
-  const real_t yz = (F(i, j + 1, k + 1) - F(i, j + 1, k - 1) - F(i, j - 1, k + 1) + F(i, j - 1, k - 1)) * f.yz;
-  const real_t gm = gmag(i, j, k);
-  const real_t den = gm * gm * gm * gm;
-  real_t out = real_t(0);
-  if (!(den < kZeroTol)) out = gauss_num(g.x(i, j, k), g.y(i, j, k), g.z(i, j, k), xx, yy, zz, xy, xz, yz) / den;
-  kappa(i, j, k) = out;
+// stencil for the mixed terms), numerator over |grad phi|^4.  Window: the
+// cross of phi on planes k-1, k, k+1.
+template <int KC>
+__global__ void __launch_bounds__(256)
+    gaussian_order2_kernel(Tiles T, real_t* __restrict__ kappa, Lay lk, const real_t* __restrict__ F, Lay lf,
+                           const real_t* __restrict__ gx, const real_t* __restrict__ gy,
+                           const real_t* __restrict__ gz, const real_t* __restrict__ gmag, Lay lg, Factors2 f) {
+  Column c;
+  if (!column(T, c)) return;
+  const int sy = lf.sy, sz = lf.sz;
+  int of = lf.at(c.i, c.j, c.k0), og = lg.at(c.i, c.j, c.k0), ok = lk.at(c.i, c.j, c.k0);
+  Cross w0 = cross_at(F, of - sz, sy), w1 = cross_at(F, of, sy), w2 = cross_at(F, of + sz, sy);
+  march<KC>(c.n, [&](int t) {
+    const int o = of + t * sz, q = og + t * lg.sz;
+    const real_t xpyp = F[o + 1 + sy], xpym = F[o + 1 - sy], xmyp = F[o - 1 + sy], xmym = F[o - 1 - sy];
+    const real_t x = gx[q], y = gy[q], z = gz[q], gm = gmag[q];
+    const Cross nx = t + 1 < c.n ? cross_at(F, o + 2 * sz, sy) : w2;
+    const real_t xx = (w1.xp - real_t(2) * w1.c + w1.xm) * f.xx;
+    const real_t yy = (w1.yp - real_t(2) * w1.c + w1.ym) * f.yy;
+    const real_t zz = (w2.c - real_t(2) * w1.c + w0.c) * f.zz;
+    const real_t xy = (xpyp - xpym - xmyp + xmym) * f.xy;
+    const real_t xz = (w2.xp - w0.xp - w2.xm + w0.xm) * f.xz;
+    const real_t yz = (w2.yp - w0.yp - w2.ym + w0.ym) * f.yz;
+    const real_t den = gm * gm * gm * gm;
+    real_t out = real_t(0);
+    if (!(den < kZeroTol)) out = gauss_num(x, y, z, xx, yy, zz, xy, xz, yz) / den;
+    kappa[ok + t * lk.sz] = out;
+    w0 = w1;
+    w1 = w2;
+    w2 = nx;
+  });
 }
 
 struct Factors4 {
@@ -170,86 +331,94 @@ struct Factors4 {
 // derivatives (:342-351): phi_xy = d/dy phi_x, phi_yz = d/dz phi_y,
 // phi_zx = d/dx phi_z.  GAUSS = 0: mean curvature over |grad phi|^3 (numerator
 // order of :358-366); GAUSS = 1: Gaussian curvature over |grad phi|^4.
-template <int GAUSS>
-__global__ void __launch_bounds__(256) curvature_order4_kernel(Box fb, View<real_t> kappa, View<const real_t> F,
-                                                               Vec3In g, View<const real_t> gmag, Factors4 f) {
-  LSMHIP_CELL(fb);
-  const real_t c = F(i, j, k);
-  const real_t xx = (-F(i + 2, j, k) + real_t(16) * F(i + 1, j, k) - real_t(30) * c - F(i - 2, j, k) +
-                     real_t(16) * F(i - 1, j, k)) * f.xx;
-  const real_t yy = (-F(i, j + 2, k) + real_t(16) * F(i, j + 1, k) - real_t(30) * c - F(i, j - 2, k) +
-                     real_t(16) * F(i, j - 1, k)) * f.yy;
-  const real_t zz = (-F(i, j, k + 2) + real_t(16) * F(i, j, k + 1) - real_t(30) * c - F(i, j, k - 2) +
-                     real_t(16) * F(i, j, k - 1)) * f.zz;
-  const real_t xy = (-g.x(i, j + 2, k) + real_t(8) * g.x(i, j + 1, k) + g.x(i, j - 2, k) -
-                     real_t(8) * g.x(i, j - 1, k)) * f.y;
-  const real_t yz = (-g.y(i, j, k + 2) + real_t(8) * g.y(i, j, k + 1) + g.y(i, j, k - 2) -
-                     real_t(8) * g.y(i, j, k - 1)) * f.z;
-  const real_t zx = (-g.z(i + 2, j, k) + real_t(8) * g.z(i + 1, j, k) + g.z(i - 2, j, k) -
-                     real_t(8) * g.z(i - 1, j, k)) * f.x;
-  const real_t gm = gmag(i, j, k);
-  const real_t den = GAUSS ? gm * gm * gm * gm : gm * gm * gm;
-  real_t out = real_t(0);
-  if (!(den < kZeroTol)) {
-    const real_t x = g.x(i, j, k), y = g.y(i, j, k), z = g.z(i, j, k);
-    const real_t num = GAUSS ? gauss_num(x, y, z, xx, yy, zz, xy, zx, yz)
-                             : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, y, z);
-    out = num / den;
-  }
-  kappa(i, j, k) = out;
+// Windows: phi and phi_y at k-2 .. k+2.
+template <int KC, int GAUSS>
+__global__ void __launch_bounds__(256)
+    curvature_order4_kernel(Tiles T, real_t* __restrict__ kappa, Lay lk, const real_t* __restrict__ F, Lay lf,
+                            const real_t* __restrict__ gx, const real_t* __restrict__ gy,
+                            const real_t* __restrict__ gz, const real_t* __restrict__ gmag, Lay lg, Factors4 f) {
+  Column c;
+  if (!column(T, c)) return;
+  const int sy = lf.sy, sz = lf.sz, gsy = lg.sy, gsz = lg.sz;
+  const int of = lf.at(c.i, c.j, c.k0), og = lg.at(c.i, c.j, c.k0), ok = lk.at(c.i, c.j, c.k0);
+  real_t p0 = F[of - 2 * sz], p1 = F[of - sz], p2 = F[of], p3 = F[of + sz], p4 = F[of + 2 * sz];
+  real_t y0 = gy[og - 2 * gsz], y1 = gy[og - gsz], y2 = gy[og], y3 = gy[og + gsz], y4 = gy[og + 2 * gsz];
+  march<KC>(c.n, [&](int t) {
+    const int o = of + t * sz, q = og + t * gsz;
+    const real_t fxm2 = F[o - 2], fxm1 = F[o - 1], fxp1 = F[o + 1], fxp2 = F[o + 2];
+    const real_t fym2 = F[o - 2 * sy], fym1 = F[o - sy], fyp1 = F[o + sy], fyp2 = F[o + 2 * sy];
+    const real_t xym2 = gx[q - 2 * gsy], xym1 = gx[q - gsy], xyp1 = gx[q + gsy], xyp2 = gx[q + 2 * gsy];
+    const real_t zxm2 = gz[q - 2], zxm1 = gz[q - 1], zxp1 = gz[q + 1], zxp2 = gz[q + 2];
+    const real_t x = gx[q], z = gz[q], gm = gmag[q];
+    const bool more = t + 1 < c.n;
+    const real_t p5 = more ? F[o + 3 * sz] : p4;
+    const real_t y5 = more ? gy[q + 3 * gsz] : y4;
+    const real_t y = y2;
+    const real_t xx = (-fxp2 + real_t(16) * fxp1 - real_t(30) * p2 - fxm2 + real_t(16) * fxm1) * f.xx;
+    const real_t yy = (-fyp2 + real_t(16) * fyp1 - real_t(30) * p2 - fym2 + real_t(16) * fym1) * f.yy;
+    const real_t zz = (-p4 + real_t(16) * p3 - real_t(30) * p2 - p0 + real_t(16) * p1) * f.zz;
+    const real_t xy = (-xyp2 + real_t(8) * xyp1 + xym2 - real_t(8) * xym1) * f.y;
+    const real_t yz = (-y4 + real_t(8) * y3 + y0 - real_t(8) * y1) * f.z;
+    const real_t zx = (-zxp2 + real_t(8) * zxp1 + zxm2 - real_t(8) * zxm1) * f.x;
+    const real_t den = GAUSS ? gm * gm * gm * gm : gm * gm * gm;
+    real_t out = real_t(0);
+    if (!(den < kZeroTol)) {
+      const real_t num = GAUSS ? gauss_num(x, y, z, xx, yy, zz, xy, zx, yz)
+                               : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, y, z);
+      out = num / den;
+    }
+    kappa[ok + t * lk.sz] = out;
+    p0 = p1; p1 = p2; p2 = p3; p3 = p4; p4 = p5;
+    y0 = y1; y1 = y2; y2 = y3; y3 = y4; y4 = y5;
+  });
 }
 
 // ---------------------------------------------------------------------------
-// Orthogonalization RHS, lsm_reinitialization3d.f.in:296-476
+// Field extension RHS, lsm_field_extension3d.f:31-156.  Window: phi at k-1,
+// k, k+1.
 // ---------------------------------------------------------------------------
-struct Sided {
-  Vec3In plus, minus;
-};
-
-__global__ void __launch_bounds__(256) orthogonalization_kernel(Box fb, View<real_t> rhs, Sided phi_d,
-                                                                View<const real_t> psi, Sided psi_d,
-                                                                real_t dx_sq) {
-  LSMHIP_CELL(fb);
-  const real_t g1 = real_t(0.5) * (psi_d.plus.x(i, j, k) + psi_d.minus.x(i, j, k));
-  const real_t g2 = real_t(0.5) * (psi_d.plus.y(i, j, k) + psi_d.minus.y(i, j, k));
-  const real_t g3 = real_t(0.5) * (psi_d.plus.z(i, j, k) + psi_d.minus.z(i, j, k));
-  const real_t ng = sqrt(g1 * g1 + g2 * g2 + g3 * g3);
-  const real_t s = psi(i, j, k);
-  real_t out = real_t(0);
-  if (fabs(s) > kZeroTol && ng > kZeroTol) {
-    const real_t sgn = s / sqrt(s * s + dx_sq * (ng * ng));
-    const real_t a1 = sgn * g1, a2 = sgn * g2, a3 = sgn * g3;
-    const real_t z = real_t(0);
-    out = real_t(-1) / ng *
-          (maxr(a1, z) * phi_d.minus.x(i, j, k) + minr(a1, z) * phi_d.plus.x(i, j, k) +
-           maxr(a2, z) * phi_d.minus.y(i, j, k) + minr(a2, z) * phi_d.plus.y(i, j, k) +
-           maxr(a3, z) * phi_d.minus.z(i, j, k) + minr(a3, z) * phi_d.plus.z(i, j, k));
-  }
-  rhs(i, j, k) = out;
-}
-
-// ---------------------------------------------------------------------------
-// Field extension RHS, lsm_field_extension3d.f:31-156
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) field_extension_kernel(Box fb, View<real_t> rhs, View<const real_t> P,
-                                                              Vec3In grad_s, Vec3In normal, real_t cutoff) {
-  LSMHIP_CELL(fb);
-  const real_t c = P(i, j, k);
-  const bool away = fabs(c) > cutoff ||
-                    (c * P(i - 1, j, k) > real_t(0) && c * P(i + 1, j, k) > real_t(0) &&
-                     c * P(i, j - 1, k) > real_t(0) && c * P(i, j + 1, k) > real_t(0) &&
-                     c * P(i, j, k - 1) > real_t(0) && c * P(i, j, k + 1) > real_t(0));
-  real_t out = real_t(0);
-  if (away)
-    out = -(normal.x(i, j, k) * grad_s.x(i, j, k) + normal.y(i, j, k) * grad_s.y(i, j, k) +
-            normal.z(i, j, k) * grad_s.z(i, j, k));
-  rhs(i, j, k) = out;
+template <int KC>
+__global__ void __launch_bounds__(256)
+    field_extension_kernel(Tiles T, real_t* __restrict__ rhs, Lay lr, const real_t* __restrict__ P, Lay lp,
+                           const real_t* __restrict__ sx, const real_t* __restrict__ sy_,
+                           const real_t* __restrict__ sz_, Lay ls, const real_t* __restrict__ nx,
+                           const real_t* __restrict__ ny, const real_t* __restrict__ nz, Lay ln, real_t cutoff) {
+  Column c;
+  if (!column(T, c)) return;
+  const int sy = lp.sy, sz = lp.sz;
+  const int op = lp.at(c.i, c.j, c.k0), os = ls.at(c.i, c.j, c.k0), on = ln.at(c.i, c.j, c.k0),
+            orr = lr.at(c.i, c.j, c.k0);
+  real_t zm = P[op - sz], c0 = P[op], zp = P[op + sz];
+  march<KC>(c.n, [&](int t) {
+    const int o = op + t * sz;
+    const real_t xm = P[o - 1], xp = P[o + 1], ym = P[o - sy], yp = P[o + sy];
+    const real_t ax = sx[os + t * ls.sz], ay = sy_[os + t * ls.sz], az = sz_[os + t * ls.sz];
+    const real_t bx = nx[on + t * ln.sz], by = ny[on + t * ln.sz], bz = nz[on + t * ln.sz];
+    const real_t next = t + 1 < c.n ? P[o + 2 * sz] : zp;
+    const bool away = fabs(c0) > cutoff || (c0 * xm > real_t(0) && c0 * xp > real_t(0) && c0 * ym > real_t(0) &&
+                                            c0 * yp > real_t(0) && c0 * zm > real_t(0) && c0 * zp > real_t(0));
+    rhs[orr + t * lr.sz] = away ? -(bx * ax + by * ay + bz * az) : real_t(0);
+    zm = c0;
+    c0 = zp;
+    zp = next;
+  });
 }
 
 Vec3In vin(const real_t* x, const real_t* y, const real_t* z, const Box& b) {
   return Vec3In{view(x, b), view(y, b), view(z, b)};
 }
 Vec3Out vout(real_t* x, real_t* y, real_t* z, const Box& b) { return Vec3Out{view(x, b), view(y, b), view(z, b)}; }
+
+// cells per thread along k
+constexpr int kPointKC = 4;   // normals, mean curvature from a Hessian
+constexpr int kOrthoKC = 2;   // 13 inputs per cell
+constexpr int kMarchKC = 16;  // stencil kernels
+
+template <int KC, class Op>
+void launch_pointwise(const Box& fb, const Op& op) {
+  const Tiles t = tiles(fb, KC);
+  hipLaunchKernelGGL((pointwise_kernel<KC, Op>), tile_grid(t), block2d(), 0, stream(), t, op);
+}
 
 }  // namespace
 
@@ -261,8 +430,7 @@ LSMHIP_EXPORT void lsm3dcomputeunitnormal_(real_t* nx, real_t* ny, real_t* nz, L
   const char* who = "lsm3dcomputeunitnormal_";
   Box fb = LSMHIP_BOX(fb), ng = LSMHIP_BOX(ng), gg = LSMHIP_BOX(gg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}})) return;
-  hipLaunchKernelGGL(unit_normal_kernel, grid_for(fb), block2d(), 0, stream(), fb, vout(nx, ny, nz, ng),
-                     vin(px, py, pz, gg));
+  launch_pointwise<kPointKC>(fb, UnitNormal{vout(nx, ny, nz, ng), vin(px, py, pz, gg)});
   finish(who);
 }
 
@@ -275,8 +443,7 @@ LSMHIP_EXPORT void lsm3dcomputesignedunitnormal_(real_t* nx, real_t* ny, real_t*
   Box fb = LSMHIP_BOX(fb), ng = LSMHIP_BOX(ng), gg = LSMHIP_BOX(gg), pg = LSMHIP_BOX(pg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}, {pg, "phi"}})) return;
   const real_t h = max3(*dx, *dy, *dz);
-  hipLaunchKernelGGL(signed_unit_normal_kernel, grid_for(fb), block2d(), 0, stream(), fb, vout(nx, ny, nz, ng),
-                     vin(px, py, pz, gg), view(phi, pg), h * h);
+  launch_pointwise<kPointKC>(fb, SignedUnitNormal{vout(nx, ny, nz, ng), vin(px, py, pz, gg), view(phi, pg), h * h});
   finish(who);
 }
 
@@ -288,9 +455,9 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvature_(real_t* kappa, LSMHIP_BOXP(kg), co
   const char* who = "lsm3dcomputemeancurvature_";
   Box fb = LSMHIP_BOX(fb), kg = LSMHIP_BOX(kg), gg = LSMHIP_BOX(gg), hg = LSMHIP_BOX(hg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}, {gg, "grad_phi"}, {hg, "hessian_phi"}})) return;
-  Hessian h{view(pxx, hg), view(pxy, hg), view(pxz, hg), view(pyy, hg), view(pyz, hg), view(pzz, hg)};
-  hipLaunchKernelGGL(mean_curvature_hessian_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(kappa, kg),
-                     vin(px, py, pz, gg), h);
+  launch_pointwise<kPointKC>(fb, MeanCurvatureHessian{view(kappa, kg), vin(px, py, pz, gg), view(pxx, hg),
+                                                      view(pxy, hg), view(pxz, hg), view(pyy, hg), view(pyz, hg),
+                                                      view(pzz, hg)});
   finish(who);
 }
 
@@ -306,8 +473,9 @@ LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder2_(real_t* kappa, LSMHIP_BO
     return;
   Factors2 f{real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
              real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz};
-  hipLaunchKernelGGL(gaussian_order2_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(kappa, kg),
-                     view(phi, pg), vin(px, py, pz, gg), view(gmag, gg), f);
+  const Tiles t = tiles(fb, kMarchKC);
+  hipLaunchKernelGGL(gaussian_order2_kernel<kMarchKC>, tile_grid(t), block2d(), 0, stream(), t, kappa, lay(kg), phi,
+                     lay(pg), px, py, pz, gmag, lay(gg), f);
   finish(who);
 }
 
@@ -320,8 +488,9 @@ static void curvature_order4(const char* who, real_t* kappa, const Box& kg, cons
     return;
   const real_t c = real_t(0.0833333333333333333333);
   Factors4 f{c / dx / dx, c / dy / dy, c / dz / dz, c / dx, c / dy, c / dz};
-  hipLaunchKernelGGL(curvature_order4_kernel<GAUSS>, grid_for(fb), block2d(), 0, stream(), fb, view(kappa, kg),
-                     view(phi, pg), vin(px, py, pz, gg), view(gmag, gg), f);
+  const Tiles t = tiles(fb, kMarchKC);
+  hipLaunchKernelGGL((curvature_order4_kernel<kMarchKC, GAUSS>), tile_grid(t), block2d(), 0, stream(), t, kappa,
+                     lay(kg), phi, lay(pg), px, py, pz, gmag, lay(gg), f);
   finish(who);
 }
 
@@ -360,10 +529,9 @@ LSMHIP_EXPORT void lsm3dcomputeorthogonalizationeqnrhs_(
                              {qm, "grad_psi_minus"}}))
     return;
   const real_t h = max3(*dx, *dy, *dz);
-  Sided phi_d{vin(pxp, pyp, pzp, gp), vin(pxm, pym, pzm, gm)};
-  Sided psi_d{vin(qxp, qyp, qzp, qp), vin(qxm, qym, qzm, qm)};
-  hipLaunchKernelGGL(orthogonalization_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(rhs, rg), phi_d,
-                     view(psi, sg), psi_d, h * h);
+  launch_pointwise<kOrthoKC>(fb, Orthogonalization{view(rhs, rg), vin(pxp, pyp, pzp, gp), vin(pxm, pym, pzm, gm),
+                                                    view(psi, sg), vin(qxp, qyp, qzp, qp), vin(qxm, qym, qzm, qm),
+                                                    h * h});
   finish(who);
 }
 
@@ -380,7 +548,8 @@ LSMHIP_EXPORT void lsm3dcomputefieldextensioneqnrhs_(real_t* rhs, LSMHIP_BOXP(rg
       !cross_inside(who, fb, pg, 1, "phi"))
     return;
   const real_t cutoff = real_t(3.0) * max3(*dx, *dy, *dz);
-  hipLaunchKernelGGL(field_extension_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(rhs, rg), view(phi, pg),
-                     vin(sx, sy, sz, ug), vin(nx, ny, nz, ng), cutoff);
+  const Tiles t = tiles(fb, kMarchKC);
+  hipLaunchKernelGGL(field_extension_kernel<kMarchKC>, tile_grid(t), block2d(), 0, stream(), t, rhs, lay(rg), phi,
+                     lay(pg), sx, sy, sz, lay(ug), nx, ny, nz, lay(ng), cutoff);
   finish(who);
 }
