@@ -1,6 +1,6 @@
 /*
  * lsmhip_config.h -- precision and argument conventions of the HIP drop-in
- * for LSMLIB's 3D level-set time-stepping kernels.
+ * for LSMLIB's level-set time-stepping kernels (3D, and their 2D twins).
  *
  * liblsmhip.so is built with LSMLIB_REAL = double and liblsmhip_f32.so with
  * LSMLIB_REAL = float; both export the same symbols, as the reference's double
@@ -11,7 +11,8 @@
  * Conventions kept from the reference (src/toolbox/lsm_spatial_derivatives3d.h:36-50
  * and every other header of the path): all arguments are pointers; arrays are
  * column-major, each addressed from the first element of its own ghost box;
- * boxes are six ints (ilo, ihi, jlo, jhi, klo, khi).  Different from the
+ * boxes are six ints (ilo, ihi, jlo, jhi, klo, khi), four (ilo, ihi, jlo, jhi)
+ * for the 2D twins.  Different from the
  * reference: array arguments are device pointers (hipMalloc'd or torch
  * tensors on the GPU); scalars, boxes, n_lo/n_hi level tables and scalar
  * outputs stay host pointers.  Calls complete before returning unless
@@ -28,6 +29,9 @@
 #define LSMHIP_BOX_ARGS(n)                                                              \
   const int *ilo_##n, const int *ihi_##n, const int *jlo_##n, const int *jhi_##n,      \
       const int *klo_##n, const int *khi_##n
+
+/* Four inclusive box bounds of a 2D array or fill box (ilo, ihi, jlo, jhi). */
+#define LSMHIP_BOX2_ARGS(n) const int *ilo_##n, const int *ihi_##n, const int *jlo_##n, const int *jhi_##n
 
 #ifdef __cplusplus
 #define LSMHIP_BEGIN_DECLS extern "C" {

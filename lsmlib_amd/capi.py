@@ -8,6 +8,7 @@ in order with a kind tag:
 
   R   real array                  (device pointer in liblsmhip)
   B   box: six ints               (host)
+  Q   2D box: four ints (ilo, ihi, jlo, jhi)  (host)
   S   real scalar, input          (host)
   SO  real scalar, output         (host)
   I   int scalar, input           (host)
@@ -176,6 +177,62 @@ _RAW = {
                                        "I:nlo_outer_minus I:nhi_outer_minus",
     "lsm3dimposemasklocal_": "R:dest B:dest R:src B:src R:mask B:mask IA:ix IA:iy IA:iz I:nlo I:nhi",
     "lsm3dcopydatalocal_": "R:dest B:dest R:src B:src IA:ix IA:iy IA:iz I:nlo I:nhi",
+    # --- 2D twins (src/*/lsm_*2d.h); arrays over (i, j), boxes of four ints ----
+    "lsm2dhjeno1_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 Q:fb S:dx S:dy",
+    "lsm2dhjeno2_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 R:D2 Q:D2 Q:fb S:dx S:dy",
+    "lsm2dhjeno3_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 R:D2 Q:D2 R:D3 Q:D3 Q:fb S:dx S:dy",
+    "lsm2dhjweno5_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 Q:fb S:dx S:dy",
+    "lsm2dupwindhjeno1_": "R:px R:py Q:g R:phi Q:phi R:vx R:vy Q:vel R:D1 Q:D1 Q:fb S:dx S:dy",
+    "lsm2dupwindhjeno2_": "R:px R:py Q:g R:phi Q:phi R:vx R:vy Q:vel R:D1 Q:D1 R:D2 Q:D2 Q:fb S:dx S:dy",
+    "lsm2dupwindhjeno3_": "R:px R:py Q:g R:phi Q:phi R:vx R:vy Q:vel R:D1 Q:D1 R:D2 Q:D2 R:D3 Q:D3 Q:fb S:dx S:dy",
+    "lsm2dupwindhjweno5_": "R:px R:py Q:g R:phi Q:phi R:vx R:vy Q:vel R:D1 Q:D1 Q:fb S:dx S:dy",
+    "lsm2dcentralgradorder2_": "R:px R:py Q:g R:phi Q:phi Q:fb S:dx S:dy",
+    "lsm2dcentralgradorder4_": "R:px R:py Q:g R:phi Q:phi Q:fb S:dx S:dy",
+    "lsm2dlaplacianorder2_": "R:lap Q:lap R:phi Q:phi Q:fb S:dx S:dy",
+    "lsm2daveragegradphi_": "R:px R:py Q:g R:pxp R:pyp Q:gp R:pxm R:pym Q:gm Q:fb",
+    "lsm2dgradientmagnitude_": "R:px R:py R:mag Q:g Q:fb",
+    "lsm2dzerooutlevelseteqnrhs_": "R:rhs Q:rhs",
+    "lsm2daddadvectiontermtolserhs_": "R:rhs Q:rhs R:px R:py Q:g R:vx R:vy Q:vel Q:fb",
+    "lsm2daddnormalveltermtolserhs_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:vn Q:vel Q:fb",
+    "lsm2daddconstnormalveltermtolserhs_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm S:vn Q:fb",
+    "lsm2daddconstcurvtermtolserhs_": "R:rhs Q:rhs R:px R:py Q:g R:pxx R:pxy R:pyy Q:g2 S:b Q:fb",
+    "lsm2daddconstprecomputedcurvtermtolserhs_": "R:rhs Q:rhs R:kappa Q:kappa R:gmag Q:gmag S:b Q:fb",
+    "lsm2daddexternalandnormalveltermtolserhs_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:vn R:vx R:vy "
+                                                 "Q:vel Q:fb",
+    "lsm2drk1step_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dtvdrk2stage1_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dtvdrk2stage2_": "R:un Q:un R:u1 Q:u1 R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dtvdrk3stage1_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dtvdrk3stage2_": "R:u2 Q:u2 R:u1 Q:u1 R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dtvdrk3stage3_": "R:un Q:un R:u2 Q:u2 R:uc Q:uc R:rhs Q:rhs Q:fb S:dt",
+    "lsm2dcomputereinitializationeqnrhs_": "R:rhs Q:rhs R:phi Q:phi R:phi0 Q:phi0 R:pxp R:pyp Q:gp R:pxm R:pym Q:gm "
+                                           "Q:fb S:dx S:dy I:use_phi0",
+    "lsm2dcomputeorthogonalizationeqnrhs_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:psi Q:psi R:qxp R:qyp "
+                                            "Q:qp R:qxm R:qym Q:qm Q:fb S:dx S:dy",
+    "lsm2dcomputemeancurvatureorder2_": "R:kappa Q:kappa R:phi Q:phi R:px R:py R:gmag Q:g Q:fb S:dx S:dy",
+    "lsm2dcomputemeancurvatureorder4_": "R:kappa Q:kappa R:phi Q:phi R:px R:py R:gmag Q:g Q:fb S:dx S:dy",
+    "lsm2dcomputeunitnormal_": "R:nx R:ny Q:nrm R:px R:py Q:g Q:fb",
+    "lsm2dcomputesignedunitnormal_": "R:nx R:ny Q:nrm R:px R:py Q:g R:phi Q:phi Q:fb S:dx S:dy",
+    "lsm2dcomputemeancurvature_": "R:kappa Q:kappa R:px R:py Q:g R:pxx R:pxy R:pyy Q:g2 Q:fb",
+    "lsm2dcomputefieldextensioneqnrhs_": "R:rhs Q:rhs R:S Q:S R:phi Q:phi R:sx R:sy Q:sg R:nx R:ny Q:nrm Q:fb "
+                                         "S:dx S:dy",
+    "lsm2dmaxnormdiff_": "SO:out R:f1 Q:f1 R:f2 Q:f2 Q:ib",
+    "lsm2dcomputestableadvectiondt_": "SO:dt R:vx R:vy Q:vel Q:ib S:dx S:dy S:cfl",
+    "lsm2dcomputestablenormalveldt_": "SO:dt R:vn Q:vel R:pxp R:pyp Q:gp R:pxm R:pym Q:gm Q:ib S:dx S:dy S:cfl",
+    "lsm2dcomputestableconstnormalveldt_": "SO:dt S:vn R:pxp R:pyp Q:gp R:pxm R:pym Q:gm Q:ib S:dx S:dy S:cfl",
+    "lsm2dvolumeintegralphilessthanzero_": "SO:int R:F Q:F R:phi Q:phi Q:ib S:dx S:dy S:eps",
+    "lsm2dvolumeintegralphigreaterthanzero_": "SO:int R:F Q:F R:phi Q:phi Q:ib S:dx S:dy S:eps",
+    "lsm2dsurfaceintegral_": "SO:int R:F Q:F R:phi Q:phi R:px R:py Q:g Q:ib S:dx S:dy S:eps",
+    "lsm2darearegionphilessthanzero_": "SO:area R:phi Q:phi Q:ib S:dx S:dy S:eps",
+    "lsm2darearegionphigreaterthanzero_": "SO:area R:phi Q:phi Q:ib S:dx S:dy S:eps",
+    "lsm2dperimeterzerolevelset_": "SO:perim R:phi Q:phi R:px R:py Q:g Q:ib S:dx S:dy S:eps",
+    "lsm2dlinearextrapolation_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dsignedlinearextrapolation_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dcopyextrapolation_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dhomogeneousneumanneno1_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dhomogeneousneumanneno2_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dhomogeneousneumanneno3_": "R:phi Q:gb Q:fb I:idx",
+    "lsm2dhomogeneousneumannweno5_": "R:phi Q:gb Q:fb I:idx",
 }
 
 # Entry points with no reference counterpart (periodic fill is absent from the
@@ -218,9 +275,9 @@ class _Fn:
             raise TypeError("expected %d arguments, got %d" % (len(self._spec), len(args)))
         keep, cargs, outs = [], [], []
         for (kind, name), a in zip(self._spec, args):
-            if kind == "B":
-                if len(a) != 6:
-                    raise ValueError("box %s needs 6 ints" % name)
+            if kind in ("B", "Q"):
+                if len(a) != (6 if kind == "B" else 4):
+                    raise ValueError("box %s needs %d ints" % (name, 6 if kind == "B" else 4))
                 for v in a:
                     c = ctypes.c_int(int(v))
                     keep.append(c)
@@ -271,11 +328,10 @@ class Bound:
             except AttributeError:
                 self.missing.append(sym)
                 continue
-            nargs = sum(6 if k == "B" else 1 for k, _ in spec)
             cfn.argtypes = None
             cfn.restype = None
             setattr(self, sym.rstrip("_"), _Fn(cfn, spec, real))
 
 
 def arg_count(sym):
-    return sum(6 if k == "B" else 1 for k, _ in SPECS[sym])
+    return sum({"B": 6, "Q": 4}.get(k, 1) for k, _ in SPECS[sym])

@@ -14,7 +14,7 @@
 // k carrying the k-neighbours in registers, so each input plane is read from
 // HBM once.  In FP64 the operation order is the reference's, without
 // contraction, and every result is bitwise the reference's except the mean
-// curvature's |grad phi|^3 = (|grad phi|^2)^1.5 (see pow_three_halves).
+// curvature's |grad phi|^3 = (|grad phi|^2)^1.5 (see stencil::pow_three_halves).
 #include "lsmhip_core.hpp"
 #include "stencil.hpp"
 
@@ -38,26 +38,6 @@ bool cross_inside(const char* who, const Box& fb, const Box& g, int w, const cha
 real_t max3(real_t a, real_t b, real_t c) {
   const real_t m = a > b ? a : b;
   return m > c ? m : c;
-}
-
-// x^1.5 for x >= 1e-11.  FP64: the exact product x * sqrt(x) carried as a
-// double-double (sqrt's remainder x - s^2 and the product's rounding error are
-// both exact with FMA) and rounded once, i.e. correctly rounded except within
-// about 2^-100 of a rounding boundary -- the value a correctly rounded pow(x,
-// 1.5) returns, which is what the C library's pow gives the reference's
-// compiled Fortran.  FP32: x * sqrt(x).
-__device__ __forceinline__ real_t pow_three_halves(real_t x) {
-  if constexpr (kDouble) {
-    const double s = sqrt(x);
-    const double p = x * s;
-    if (__builtin_isinf(p)) return p;
-    const double e = fma(x, s, -p);       // x s - p
-    const double r = fma(-s, s, x);       // x - s^2
-    const double t = r / (2.0 * s);       // sqrt(x) - s to second order
-    return p + fma(x, t, e);
-  } else {
-    return x * sqrt(x);
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -201,7 +181,7 @@ struct MeanCurvatureHessian {
       const real_t num = v.xx * y * y - real_t(2) * v.xy * x * y + v.yy * x * x + v.xx * z * z -
                          real_t(2) * v.xz * x * z + v.zz * x * x + v.yy * z * z - real_t(2) * v.yz * y * z +
                          v.zz * y * y;
-      out = num / pow_three_halves(ngs);
+      out = num / stencil::pow_three_halves(ngs);
     }
     kappa(i, j, k) = out;
   }

@@ -198,6 +198,27 @@ __device__ __forceinline__ T curv_num(T xx, T xy, T xz, T yy, T yz, T zz, T x, T
          zz * y * y - T(2) * yz * y * z;
 }
 
+// x^1.5 for x >= 1e-11.  FP64: the exact product x * sqrt(x) carried as a
+// double-double (sqrt's remainder x - s^2 and the product's rounding error are
+// both exact with FMA) and rounded once, i.e. correctly rounded except within
+// about 2^-100 of a rounding boundary -- the value a correctly rounded pow(x,
+// 1.5) returns, which is what the C library's pow gives the reference's
+// compiled Fortran.  FP32: x * sqrt(x).
+template <class T>
+__device__ __forceinline__ T pow_three_halves(T x) {
+  if constexpr (sizeof(T) == 8) {
+    const double s = sqrt(x);
+    const double p = x * s;
+    if (__builtin_isinf(p)) return p;
+    const double e = fma(x, s, -p);       // x s - p
+    const double r = fma(-s, s, x);       // x - s^2
+    const double t = r / (2.0 * s);       // sqrt(x) - s to second order
+    return p + fma(x, t, e);
+  } else {
+    return x * sqrt(x);
+  }
+}
+
 // Per-axis Godunov choice for external + normal motion,
 // lsm_level_set_evolution3d.f.in:629-693.
 template <class T>

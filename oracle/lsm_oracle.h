@@ -68,6 +68,21 @@ static inline orc_mask orc_mview(const unsigned char *p, const int *ilo, const i
 #define AT(a, i, j, k) \
   ((a).p[((long)(i) - (a).ilo) + (a).nx * ((long)(j) - (a).jlo) + (a).nxy * ((long)(k) - (a).klo)])
 
+/* An inclusive box; 2D boxes have klo = khi = 0. */
+typedef struct {
+  int ilo, ihi, jlo, jhi, klo, khi;
+} obox;
+
+/* Shared between the 3D and 2D restatements (oracle_wholegrid.c): the
+ * plus/minus and velocity-upwinded HJ-ENO1/2/3 (order 1..3) and HJ-WENO5
+ * (order 5) drivers over the first nd axes, and one face of a ghost fill. */
+void orc_hj_plus_minus(int nd, int order, orc_arr *plus, orc_arr *minus, orc_arr phi, orc_arr D1, obox gb1,
+                       orc_arr D2, obox gb2, orc_arr D3, obox gb3, obox fb, const double inv[3]);
+void orc_hj_upwind(int nd, int order, orc_arr *out, orc_arr phi, orc_arr *vel, orc_arr D1, obox gb1, orc_arr D2,
+                   obox gb2, orc_arr D3, obox gb3, obox fb, const double inv[3]);
+enum { BC_LINEAR = 0, BC_SIGNED = 1, BC_COPY = 2 };
+void orc_bc_face(orc_arr P, obox gb, obox fb, int idx, int kind);
+
 /* Fortran integer*1 marks are compared as signed bytes. */
 #define MARK(a, i, j, k) ((signed char)AT(a, i, j, k))
 

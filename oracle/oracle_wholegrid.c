@@ -10,10 +10,6 @@
  */
 #include "lsm_oracle.h"
 
-typedef struct {
-  int ilo, ihi, jlo, jhi, klo, khi;
-} obox;
-
 static obox mkbox(const int *ilo, const int *ihi, const int *jlo, const int *jhi,
                   const int *klo, const int *khi) {
   obox b = {*ilo, *ihi, *jlo, *jhi, *klo, *khi};
@@ -159,9 +155,9 @@ static double weno5_minus(orc_arr D1, int d, int i, int j, int k, double inv) {
  * (src/toolbox/lsm_spatial_derivatives3d.f.in:199-1611): for each axis compute
  * the needed differences over the widened fill box, then the two one-sided
  * derivatives on the fill box. */
-static void hj_plus_minus(int order, orc_arr *plus, orc_arr *minus, orc_arr phi, orc_arr D1, obox gb1,
+void orc_hj_plus_minus(int nd, int order, orc_arr *plus, orc_arr *minus, orc_arr phi, orc_arr D1, obox gb1,
                           orc_arr D2, obox gb2, orc_arr D3, obox gb3, obox fb, const double inv[3]) {
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < nd; d++) {
     if (order == 1) {
       computeDn(D1, gb1, phi, fb, 1, d);
     } else if (order == 2) {
@@ -206,7 +202,7 @@ void oracle_lsm3dhjeno1(double *pxp, double *pyp, double *pzp, BOXP(gp), double 
   orc_arr M[3] = {VIEW(pxm, gm), VIEW(pym, gm), VIEW(pzm, gm)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_plus_minus(1, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
+  orc_hj_plus_minus(3, 1, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
 }
 
 /* lsm3dHJENO2, src/toolbox/lsm_spatial_derivatives3d.f.in:393-663 */
@@ -219,7 +215,7 @@ void oracle_lsm3dhjeno2(double *pxp, double *pyp, double *pzp, BOXP(gp), double 
   orc_arr M[3] = {VIEW(pxm, gm), VIEW(pym, gm), VIEW(pzm, gm)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_plus_minus(2, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), none, nb,
+  orc_hj_plus_minus(3, 2, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), none, nb,
                 BOX(fb), inv);
 }
 
@@ -231,7 +227,7 @@ void oracle_lsm3dhjeno3(double *pxp, double *pyp, double *pzp, BOXP(gp), double 
   INV3(inv);
   orc_arr P[3] = {VIEW(pxp, gp), VIEW(pyp, gp), VIEW(pzp, gp)};
   orc_arr M[3] = {VIEW(pxm, gm), VIEW(pym, gm), VIEW(pzm, gm)};
-  hj_plus_minus(3, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), VIEW(D3, d3),
+  orc_hj_plus_minus(3, 3, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), VIEW(D3, d3),
                 BOX(d3), BOX(fb), inv);
 }
 
@@ -244,15 +240,15 @@ void oracle_lsm3dhjweno5(double *pxp, double *pyp, double *pzp, BOXP(gp), double
   orc_arr M[3] = {VIEW(pxm, gm), VIEW(pym, gm), VIEW(pzm, gm)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_plus_minus(5, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
+  orc_hj_plus_minus(3, 5, P, M, VIEW(phi, phi_gb), VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
 }
 
 /* Shared driver of the velocity-upwinded routines
  * (src/toolbox/lsm_spatial_derivatives3d.f.in:1630-2999): |v| < zero_tol gives
  * 0, v > 0 the minus (backward) derivative, otherwise the plus one. */
-static void hj_upwind(int order, orc_arr *out, orc_arr phi, orc_arr *vel, orc_arr D1, obox gb1, orc_arr D2,
+void orc_hj_upwind(int nd, int order, orc_arr *out, orc_arr phi, orc_arr *vel, orc_arr D1, obox gb1, orc_arr D2,
                       obox gb2, orc_arr D3, obox gb3, obox fb, const double inv[3]) {
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < nd; d++) {
     if (order == 1) {
       computeDn(D1, gb1, phi, fb, 1, d);
     } else if (order == 2) {
@@ -294,7 +290,7 @@ void oracle_lsm3dupwindhjeno1(double *px, double *py, double *pz, BOXP(g), const
   orc_arr V[3] = {VIEW(vx, vel_gb), VIEW(vy, vel_gb), VIEW(vz, vel_gb)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_upwind(1, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
+  orc_hj_upwind(3, 1, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
 }
 
 /* lsm3dUpwindHJENO2, src/toolbox/lsm_spatial_derivatives3d.f.in:1854-2156 */
@@ -307,7 +303,7 @@ void oracle_lsm3dupwindhjeno2(double *px, double *py, double *pz, BOXP(g), const
   orc_arr V[3] = {VIEW(vx, vel_gb), VIEW(vy, vel_gb), VIEW(vz, vel_gb)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_upwind(2, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), none, nb, BOX(fb), inv);
+  orc_hj_upwind(3, 2, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), none, nb, BOX(fb), inv);
 }
 
 /* lsm3dUpwindHJENO3, src/toolbox/lsm_spatial_derivatives3d.f.in:2174-2609 */
@@ -318,7 +314,7 @@ void oracle_lsm3dupwindhjeno3(double *px, double *py, double *pz, BOXP(g), const
   INV3(inv);
   orc_arr O[3] = {VIEW(px, g), VIEW(py, g), VIEW(pz, g)};
   orc_arr V[3] = {VIEW(vx, vel_gb), VIEW(vy, vel_gb), VIEW(vz, vel_gb)};
-  hj_upwind(3, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), VIEW(D3, d3), BOX(d3),
+  orc_hj_upwind(3, 3, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), VIEW(D2, d2), BOX(d2), VIEW(D3, d3), BOX(d3),
             BOX(fb), inv);
 }
 
@@ -332,7 +328,7 @@ void oracle_lsm3dupwindhjweno5(double *px, double *py, double *pz, BOXP(g), cons
   orc_arr V[3] = {VIEW(vx, vel_gb), VIEW(vy, vel_gb), VIEW(vz, vel_gb)};
   orc_arr none = {0};
   obox nb = {0};
-  hj_upwind(5, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
+  orc_hj_upwind(3, 5, O, VIEW(phi, phi_gb), V, VIEW(D1, d1), BOX(d1), none, nb, none, nb, BOX(fb), inv);
 }
 
 /* lsm3dCentralGradOrder2, src/toolbox/lsm_spatial_derivatives3d.f.in:3016-3077 */
@@ -871,9 +867,8 @@ void oracle_lsm3dsurfaceareazerolevelset(double *area, const double *phi, BOXP(p
  * Ghost-cell fills, src/boundary_conditions/lsm_boundary_conditions3d.f:46-638
  * Each face sweeps the full ghost box of the other two axes.
  * ------------------------------------------------------------------------- */
-enum { BC_LINEAR = 0, BC_SIGNED = 1, BC_COPY = 2 };
 
-static void bc_face(orc_arr P, obox gb, obox fb, int idx, int kind) {
+void orc_bc_face(orc_arr P, obox gb, obox fb, int idx, int kind) {
   int d = idx / 2, hi = idx % 2;
   int lo_g[3] = {gb.ilo, gb.jlo, gb.klo}, hi_g[3] = {gb.ihi, gb.jhi, gb.khi};
   int lo_f[3] = {fb.ilo, fb.jlo, fb.klo}, hi_f[3] = {fb.ihi, fb.jhi, fb.khi};
@@ -904,13 +899,13 @@ static void bc_face(orc_arr P, obox gb, obox fb, int idx, int kind) {
 }
 
 void oracle_lsm3dlinearextrapolation(double *phi, BOXP(gb), BOXP(fb), const int *idx) {
-  if (*idx >= 0 && *idx <= 5) bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_LINEAR);
+  if (*idx >= 0 && *idx <= 5) orc_bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_LINEAR);
 }
 void oracle_lsm3dsignedlinearextrapolation(double *phi, BOXP(gb), BOXP(fb), const int *idx) {
-  if (*idx >= 0 && *idx <= 5) bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_SIGNED);
+  if (*idx >= 0 && *idx <= 5) orc_bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_SIGNED);
 }
 void oracle_lsm3dcopyextrapolation(double *phi, BOXP(gb), BOXP(fb), const int *idx) {
-  if (*idx >= 0 && *idx <= 5) bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_COPY);
+  if (*idx >= 0 && *idx <= 5) orc_bc_face(VIEW(phi, gb), BOX(gb), BOX(fb), *idx, BC_COPY);
 }
 /* lsm_boundary_conditions3d.f:486-638 -- all four are copy extrapolation */
 void oracle_lsm3dhomogeneousneumanneno1(double *phi, BOXP(gb), BOXP(fb), const int *idx) {

@@ -36,6 +36,9 @@ def reference():
 
 
 def shape_of(box):
+    if len(box) == 4:  # 2D box (ilo, ihi, jlo, jhi)
+        ilo, ihi, jlo, jhi = box
+        return (jhi - jlo + 1, ihi - ilo + 1)
     ilo, ihi, jlo, jhi, klo, khi = box
     return (khi - klo + 1, jhi - jlo + 1, ihi - ilo + 1)  # C order: i fastest (column-major)
 
@@ -48,7 +51,7 @@ def box_of_array(sym):
     for kind, name in spec:
         if kind in ("R", "M"):
             pending.append(name)
-        elif kind == "B":
+        elif kind in ("B", "Q"):
             for p in pending:
                 out[p] = name
             pending = []
@@ -115,7 +118,7 @@ def run(impl, sym, case, device=None, skip_arrays=(), real32=False):
             if name not in arrs:
                 arrs[name] = np.array(case.arrays[name], copy=True)
             args.append(arrs[name])
-        elif kind == "B":
+        elif kind in ("B", "Q"):
             args.append(case.box(name))
         elif kind in ("S", "I", "U8"):
             args.append(case.scalars[name])
@@ -180,6 +183,34 @@ def grid_case(seed=0, gb=(-3, 20, -3, 16, -3, 12), fb=(0, 17, 0, 13, 0, 9), mixe
     flat[sel[len(sel) // 2:]] *= 1e-12
     c.arrays["phi"] = phi
     for v in ("vx", "vy", "vz", "vn"):
+        a = rng.standard_normal(shape_of(c.box("vel")))
+        f = a.reshape(-1)
+        s = rng.choice(f.size, f.size // 10, replace=False)
+        f[s[: len(s) // 2]] = 0.0
+        f[s[len(s) // 2:]] *= 1e-12
+        c.arrays[v] = a
+    return c
+
+
+def grid_case2d(seed=0, gb=(-3, 22, -3, 18), fb=(0, 19, 0, 15), mixed_boxes=True):
+    """The 2D counterpart of grid_case: random fields over (i, j) with exact
+    zeros and tiny values, output and velocity arrays on ghost boxes of their
+    own when mixed_boxes."""
+    c = Case(gb, fb, seed)
+    if mixed_boxes:
+        c.boxes["gp"] = (gb[0] + 1, gb[1] - 1, gb[2] + 2, gb[3])
+        c.boxes["gm"] = (gb[0], gb[1] - 2, gb[2] + 1, gb[3] - 1)
+        c.boxes["g"] = (gb[0] + 2, gb[1], gb[2], gb[3] - 2)
+        c.boxes["vel"] = (gb[0] + 1, gb[1], gb[2] + 1, gb[3])
+        c.boxes["rhs"] = (gb[0] + 2, gb[1] - 1, gb[2] + 1, gb[3] - 2)
+    rng = c.rng
+    phi = rng.standard_normal(shape_of(gb))
+    flat = phi.reshape(-1)
+    sel = rng.choice(flat.size, flat.size // 20, replace=False)
+    flat[sel[: len(sel) // 2]] = 0.0
+    flat[sel[len(sel) // 2:]] *= 1e-12
+    c.arrays["phi"] = phi
+    for v in ("vx", "vy", "vn"):
         a = rng.standard_normal(shape_of(c.box("vel")))
         f = a.reshape(-1)
         s = rng.choice(f.size, f.size // 10, replace=False)

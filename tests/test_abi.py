@@ -19,7 +19,7 @@ def declared_symbols():
     syms = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         txt = open(h).read()
-        macros = dict(re.findall(r"#define\s+(LSM3D_\w+)\s+(\w+_)\s*$", txt, flags=re.M))
+        macros = dict(re.findall(r"#define\s+(LSM[23]D_\w+)\s+(\w+_)\s*$", txt, flags=re.M))
         syms.update(macros.values())
         for name in re.findall(r"^(?:int|void|const char \*|void \*)\s*\*?\s*(lsm\w+)\(", txt, flags=re.M):
             syms.add(name)
@@ -77,13 +77,16 @@ _KIND_TYPE = {"R": ("double*", "constdouble*"), "S": ("constdouble*",), "SO": ("
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs the C preprocessor")
 def test_prototypes_match_the_ctypes_table():
     """Every drop-in prototype has, position by position, the parameter kind
-    the ctypes table passes (boxes are six const int pointers)."""
+    the ctypes table passes (boxes are six const int pointers, 2D boxes four)."""
     protos = _expanded_prototypes()
     for sym, spec in capi.SPECS.items():
         assert sym in protos, sym
         want = []
         for kind, name in spec:
-            want.extend([("constint*",)] * 6 if kind == "B" else [_KIND_TYPE[kind]])
+            if kind in ("B", "Q"):
+                want.extend([("constint*",)] * (6 if kind == "B" else 4))
+            else:
+                want.append(_KIND_TYPE[kind])
         got = protos[sym]
         assert len(got) == len(want), (sym, len(got), len(want))
         for pos, (g, w) in enumerate(zip(got, want)):
