@@ -233,6 +233,64 @@ _RAW = {
     "lsm2dhomogeneousneumanneno2_": "R:phi Q:gb Q:fb I:idx",
     "lsm2dhomogeneousneumanneno3_": "R:phi Q:gb Q:fb I:idx",
     "lsm2dhomogeneousneumannweno5_": "R:phi Q:gb Q:fb I:idx",
+    # --- 2D narrow-band twins (src/*/lsm_*2d_local.h, lsm_localization2d.h) -----
+    "lsm2dhjeno1local_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 S:dx S:dy IA:ix IA:iy "
+                         "I:nlo0 I:nhi0 I:nlo1 I:nhi1 M:nb Q:nb U8:mark_fb U8:mark_D1",
+    "lsm2dhjeno2local_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 R:D2 Q:D2 S:dx S:dy IA:ix IA:iy "
+                         "I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 M:nb Q:nb U8:mark_fb U8:mark_D1 U8:mark_D2",
+    "lsm2dhjeno3local_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 R:D2 Q:D2 R:D3 Q:D3 S:dx S:dy "
+                         "IA:ix IA:iy I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 I:nlo3 I:nhi3 M:nb Q:nb "
+                         "U8:mark_fb U8:mark_D1 U8:mark_D2 U8:mark_D3",
+    "lsm2dhjweno5local_": "R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:phi Q:phi R:D1 Q:D1 S:dx S:dy IA:ix IA:iy "
+                          "I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 I:nlo3 I:nhi3 M:nb Q:nb U8:mark_fb U8:mark_D1",
+    "lsm2dupwindhjeno2local_": "R:px R:py Q:g R:phi Q:phi R:vx R:vy Q:vel R:D1 Q:D1 R:D2 Q:D2 S:dx S:dy IA:ix IA:iy "
+                               "I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 M:nb Q:nb U8:mark_fb U8:mark_D1 "
+                               "U8:mark_D2",
+    "lsm2dcentralgradorder2local_": "R:px R:py Q:g R:phi Q:phi S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcentralgradorder4local_": "R:px R:py Q:g R:phi Q:phi S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dlaplacianorder2local_": "R:lap Q:lap R:phi Q:phi S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputeavegradphilocal_": "SO:ave R:phi Q:phi S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dgradientmagnitudelocal_": "R:px R:py R:mag Q:g IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dzerooutlevelseteqnrhslocal_": "R:rhs Q:rhs IA:ix IA:iy I:nlo I:nhi",
+    "lsm2daddadvectiontermtolserhslocal_": "R:rhs Q:rhs R:px R:py Q:g R:vx R:vy Q:vel IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2daddnormalveltermtolserhslocal_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:vn Q:vel IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2daddconstnormalveltermtolserhslocal_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm S:vn IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2daddconstcurvtermtolserhslocal_": "R:rhs Q:rhs R:px R:py Q:g R:pxx R:pxy R:pyy Q:g2 S:b IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2daddconstprecomputedcurvtermtolserhslocal_": "R:rhs Q:rhs R:kappa Q:kappa R:gmag Q:gmag S:b IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2daddexternalandnormalveltermtolserhslocal_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:vn R:vx R:vy "
+                                                      "Q:vel IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2drk1steplocal_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dtvdrk2stage1local_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dtvdrk2stage2local_": "R:un Q:un R:u1 Q:u1 R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dtvdrk3stage1local_": "R:un Q:un R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dtvdrk3stage2local_": "R:u2 Q:u2 R:u1 Q:u1 R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dtvdrk3stage3local_": "R:un Q:un R:u2 Q:u2 R:uc Q:uc R:rhs Q:rhs S:dt IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dmaxnormdifflocal_": "SO:out R:f1 Q:f1 R:f2 Q:f2 IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputestableadvectiondtlocal_": "SO:dt R:vx R:vy Q:vel S:dx S:dy S:cfl IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputestablenormalveldtlocal_": "SO:dt R:vn Q:vel R:pxp R:pyp Q:gp R:pxm R:pym Q:gm S:dx S:dy S:cfl "
+                                           "IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputestableconstnormalveldtlocal_": "SO:dt S:vn R:pxp R:pyp Q:gp R:pxm R:pym Q:gm S:dx S:dy S:cfl "
+                                                "IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputereinitializationeqnrhslocal_": "R:rhs Q:rhs R:phi Q:phi R:phi0 Q:phi0 R:pxp R:pyp Q:gp R:pxm R:pym "
+                                                "Q:gm S:dx S:dy I:use_phi0 IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputeorthogonalizationeqnrhslocal_": "R:rhs Q:rhs R:pxp R:pyp Q:gp R:pxm R:pym Q:gm R:psi Q:psi R:qxp "
+                                                 "R:qyp Q:qp R:qxm R:qym Q:qm S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputemeancurvatureorder2local_": "R:kappa Q:kappa R:phi Q:phi R:px R:py R:gmag Q:g S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputemeancurvatureorder4local_": "R:kappa Q:kappa R:phi Q:phi R:px R:py R:gmag Q:g S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputeunitnormallocal_": "R:nx R:ny Q:nrm R:px R:py Q:g S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2dcomputesignedunitnormallocal_": "R:nx R:ny Q:nrm R:px R:py Q:g R:phi Q:phi S:dx S:dy IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb",
+    "lsm2ddeterminenarrowband_": "R:phi Q:phi M:nb Q:nb IA:ix IA:iy I:nlo I:nhi IH:n_lo IH:n_hi IA:outer "
+                                 "I:nlo_outer I:nhi_outer IO:nlo_outer_plus IO:nhi_outer_plus IO:nlo_outer_minus "
+                                 "IO:nhi_outer_minus S:width S:width_inner I:level",
+    "lsm2dmarknarrowbandboundarylayer_": "M:nb Q:gb Q:fb U8:mark",
+    "lsm2ddeterminenarrowbandfrommask_": "R:mask Q:mask M:nb Q:nb IA:ix IA:iy I:nlo I:nhi IH:n_lo IH:n_hi "
+                                         "I:level I:use_mask_sign",
+    "lsm2dmultiplycutofflserhslocal_": "R:phi R:rhs Q:gb IA:ix IA:iy I:nlo I:nhi M:nb Q:nb U8:mark_fb S:beta S:gamma",
+    "lsm2dcheckouternarrowbandlayer_": "IO:change_sign R:phi Q:gb IA:ix IA:iy I:nlo I:nhi IA:outer "
+                                       "I:nlo_outer I:nhi_outer I:nlo_outer_plus I:nhi_outer_plus "
+                                       "I:nlo_outer_minus I:nhi_outer_minus",
+    "lsm2dimposemasklocal_": "R:dest Q:dest R:src Q:src R:mask Q:mask IA:ix IA:iy I:nlo I:nhi",
+    "lsm2dcopydatalocal_": "R:dest Q:dest R:src Q:src IA:ix IA:iy I:nlo I:nhi",
 }
 
 # Entry points with no reference counterpart (periodic fill is absent from the

@@ -1,6 +1,8 @@
 // band3d.hip -- narrow-band construction and the band checks around it
-// (src/toolbox/lsm_localization3d.f:48-653), with index lists bit-exact to
-// the reference's sequential loops.
+// (src/toolbox/lsm_localization3d.f:48-653, and the 2D twins of
+// src/toolbox/lsm_localization2d.f:48-772), with index lists bit-exact to the
+// reference's sequential loops.  A 2D band is a 3D band of one plane (k = 0)
+// with no index_z list and four neighbour directions.
 //
 // Level 0 (lsm3dDetermineNarrowBand, :174-297) is the set |phi| < width in
 // (k,j,i)-lexicographic order of phi's ghost box, i.e. memory order: an
@@ -63,7 +65,7 @@ struct Level0 {
   int nx, ny;
   Box B;
   real_t width, width_inner;
-  int mode;           // 0: |phi| < width; 1: mask >= 0; 2: mask <= 0
+  int mode;           // 0: |phi| < width; 1: mask >= 0; 2: mask <= 0; 3: mask < 0
   __device__ void flags(long long t, bool f[3]) const {
     f[0] = f[1] = f[2] = false;
     if (t >= n) return;
@@ -75,7 +77,7 @@ struct Level0 {
       f[2] = outer && !(v <= real_t(0));
     } else {
       const real_t v = mask[t];
-      f[0] = mode == 1 ? v >= real_t(0) : v <= real_t(0);
+      f[0] = mode == 1 ? v >= real_t(0) : (mode == 2 ? v <= real_t(0) : v < real_t(0));
     }
   }
 };
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(1024) scan3_kernel(int* v, long long n, long l
 }
 
 struct Level0Out {
-  int *ix, *iy, *iz;  // element 0 is index nlo
+  int *ix, *iy, *iz;  // element 0 is index nlo; iz null for a 2D band
   int nlo;
   int* outer;         // element 0 is index nlo_outer
   int nlo_outer, nhi_outer;
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* o
         const int r = run[0] + pre[0];
         o.ix[r] = i;
         o.iy[r] = j;
-        o.iz[r] = k;
+        if (o.iz) o.iz[r] = k;
         if (f[1]) o.outer[run[1] + pre[1]] = o.nlo + r;
         if (f[2]) o.outer[(o.nhi_outer - o.nlo_outer) - (run[2] + pre[2])] = o.nlo + r;
       }
@@ -170,10 +172,11 @@ __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* o
 
 // ----- levels >= 1 ------------------------------------------------------------------------------
 struct Grow {
-  const int *ix, *iy, *iz;  // element 0 is index base
+  const int *ix, *iy, *iz;  // element 0 is index base; iz null for a 2D band
   int base;
   int plo;                  // first parent index
-  long long ncand;          // 6 * parents
+  int ndir;                 // neighbour directions: 6 in 3D, 4 in 2D
+  long long ncand;          // ndir * parents
   unsigned char* nb;
   Box nbb;
   int* claim;               // over nbb
@@ -181,8 +184,8 @@ struct Grow {
 
 __device__ __forceinline__ bool target(const Grow& g, long long c, int& off, int& qi, int& qj, int& qk, int* flag) {
   if (c >= g.ncand) return false;
-  const int m = g.plo + (int)(c / 6), d = (int)(c % 6);
-  const int i = g.ix[m - g.base], j = g.iy[m - g.base], k = g.iz[m - g.base];
+  const int m = g.plo + (int)(c / g.ndir), d = (int)(c % g.ndir);
+  const int i = g.ix[m - g.base], j = g.iy[m - g.base], k = g.iz ? g.iz[m - g.base] : 0;
   const Box& b = g.nbb;
   if (!b.has(i, j, k)) {
     *flag = 1;
@@ -246,7 +249,7 @@ __global__ void __launch_bounds__(kT) grow_write_kernel(Grow g, const int* offse
       const int r = start_rel + run + pre[0];
       ox[r] = qi;
       oy[r] = qj;
-      oz[r] = qk;
+      if (oz) oz[r] = qk;
       g.nb[off] = mark;
       g.claim[off] = kNoClaim;
     }
@@ -290,8 +293,10 @@ bool read_totals(const char* who, long long tot[3]) {
 }
 
 // Grows levels 1..level from level 0 (n_lo/n_hi are host arrays of the caller).
+// iz null: a 2D band, grown along +x, -x, +y, -y.
 bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, int* iy, int* iz, int nlo, int nhi,
                  int* n_lo, int* n_hi, int level) {
+  const int ndir = iz ? 6 : 4;
   if (level < 1) return true;
   const long long ncells = nbb.volume();
   long long maxcand = 0;
@@ -307,11 +312,11 @@ bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, in
       continue;
     }
     const long long parents = (long long)n_hi[L - 1] - n_lo[L - 1] + 1;
-    const long long ncand = 6 * parents;
+    const long long ncand = ndir * parents;
     const long long nblk = (ncand + kChunk - 1) / kChunk;
     if (ncand > maxcand) maxcand = ncand;
     if (!ensure(who, ncells, nblk)) return false;
-    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ncand, nb, nbb, g_claim};
+    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ndir, ncand, nb, nbb, g_claim};
     hipLaunchKernelGGL(claim_kernel, dim3((unsigned)((ncand + kT - 1) / kT)), dim3(kT), 0, stream(), g, flag);
     hipLaunchKernelGGL(grow_count_kernel, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, flag);
     hipLaunchKernelGGL(scan3_kernel, dim3(1), dim3(1024), 0, stream(), g_counts, nblk, g_tot);
@@ -442,11 +447,9 @@ __global__ void __launch_bounds__(256) boundary_layer_kernel(unsigned char* nb, 
 }
 }  // namespace
 
-// :303-377 -- six fill-box planes, each spanning the ghost box of the other axes.
-LSMHIP_EXPORT void lsm3dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb),
-                                                     const unsigned char* mark) {
-  const char* who = "lsm3dmarknarrowbandboundarylayer_";
-  const Box gb = LSMHIP_BOX(gb), fb = LSMHIP_BOX(fb);
+// The 2*nd fill-box planes (lines in 2D), each spanning the ghost box of the other axes.
+static void mark_boundary(const char* who, unsigned char* nb, const Box& gb, const Box& fb, unsigned char mark,
+                          int nd) {
   if (gb.empty() || fb.empty()) return;
   // every plane index must lie in the ghost box (the reference writes them unchecked)
   if (fb.ilo < gb.ilo || fb.ihi > gb.ihi || fb.jlo < gb.jlo || fb.jhi > gb.jhi || fb.klo < gb.klo || fb.khi > gb.khi) {
@@ -454,13 +457,19 @@ LSMHIP_EXPORT void lsm3dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_B
     return;
   }
   if (!check_size(gb, who)) return;
-  for (int d = 0; d < 6; d++) {
+  for (int d = 0; d < 2 * nd; d++) {
     const int ax = d / 2;
     const long long n = (long long)(ax == 0 ? gb.ny() * gb.nz() : ax == 1 ? gb.nx() * gb.nz() : gb.nx() * gb.ny());
     hipLaunchKernelGGL(boundary_layer_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream(), nb, gb, fb,
-                       *mark, d);
+                       mark, d);
   }
   finish(who);
+}
+
+// :303-377 -- six fill-box planes, each spanning the ghost box of the other axes.
+LSMHIP_EXPORT void lsm3dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb),
+                                                     const unsigned char* mark) {
+  mark_boundary("lsm3dmarknarrowbandboundarylayer_", nb, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *mark, 3);
 }
 
 namespace {
@@ -475,7 +484,7 @@ __global__ void __launch_bounds__(256) outer_sign_kernel(const real_t* phi, Box 
     *flag = 1;
     return;
   }
-  const int i = ix[l - nlo], j = iy[l - nlo], k = iz[l - nlo];
+  const int i = ix[l - nlo], j = iy[l - nlo], k = iz ? iz[l - nlo] : 0;
   if (!g.has(i, j, k)) {
     *flag = 1;
     return;
@@ -487,19 +496,15 @@ __global__ void __launch_bounds__(256) outer_sign_kernel(const real_t* phi, Box 
 
 // :568-653 -- change_sign = 1 when the outer plus list or the outer minus list
 // holds points of both signs (the reference's early exits do not change this).
-LSMHIP_EXPORT void lsm3dcheckouternarrowbandlayer_(int* change_sign, const real_t* phi, LSMHIP_BOXP(g), const int* ix,
-                                                   const int* iy, const int* iz, const int* nlo, const int* nhi,
-                                                   const int* outer, const int* nlo_outer, const int* nhi_outer,
-                                                   const int* nlo_plus, const int* nhi_plus, const int* nlo_minus,
-                                                   const int* nhi_minus) {
-  const char* who = "lsm3dcheckouternarrowbandlayer_";
-  (void)nhi_outer;
+static void check_outer(const char* who, int* change_sign, const real_t* phi, const Box& g, const int* ix,
+                        const int* iy, const int* iz, const int* nlo, const int* nhi, const int* outer,
+                        const int* nlo_outer, const int* nhi_outer, const int* nlo_plus, const int* nhi_plus,
+                        const int* nlo_minus, const int* nhi_minus) {
   *change_sign = 0;
   int* bits = (int*)workspace(2 * sizeof(int));
   int* flag = device_flag();
   if (!bits || !flag) return;
   if (!check_hip(hipMemsetAsync(bits, 0, 2 * sizeof(int), stream()), who)) return;
-  const Box g = LSMHIP_BOX(g);
   const int lo[2] = {*nlo_plus, *nlo_minus}, hi[2] = {*nhi_plus, *nhi_minus};
   for (int w = 0; w < 2; w++) {
     if (hi[w] < lo[w]) continue;
@@ -516,4 +521,96 @@ LSMHIP_EXPORT void lsm3dcheckouternarrowbandlayer_(int* change_sign, const real_
   if (!check_hip(hipStreamSynchronize(stream()), who)) return;
   take_device_flag(who);
   *change_sign = (h[0] == 3 || h[1] == 3) ? 1 : 0;
+}
+
+LSMHIP_EXPORT void lsm3dcheckouternarrowbandlayer_(int* change_sign, const real_t* phi, LSMHIP_BOXP(g), const int* ix,
+                                                   const int* iy, const int* iz, const int* nlo, const int* nhi,
+                                                   const int* outer, const int* nlo_outer, const int* nhi_outer,
+                                                   const int* nlo_plus, const int* nhi_plus, const int* nlo_minus,
+                                                   const int* nhi_minus) {
+  check_outer("lsm3dcheckouternarrowbandlayer_", change_sign, phi, LSMHIP_BOX(g), ix, iy, iz, nlo, nhi, outer,
+              nlo_outer, nhi_outer, nlo_plus, nhi_plus, nlo_minus, nhi_minus);
+}
+
+// ---------------------------------------------------------------------------
+// src/toolbox/lsm_localization2d.h entry points (2D boxes of four ints).
+// ---------------------------------------------------------------------------
+#define LSMHIP_BOXP2(n) const int *ilo_##n, const int *ihi_##n, const int *jlo_##n, const int *jhi_##n
+#define LSMHIP_BOX2(n) ::lsmhip::Box{*ilo_##n, *ihi_##n, *jlo_##n, *jhi_##n, 0, 0}
+
+// lsm2dDetermineNarrowBand, lsm_localization2d.f:139-243.  Unlike the 3D
+// routine, the counters are stored whatever the band holds (n_hi(0) = nlo - 1
+// for an empty band), and levels 1..level grow only when level 0 holds at
+// least two points (:231); otherwise n_lo/n_hi(1..level) are left as they were.
+LSMHIP_EXPORT void lsm2ddeterminenarrowband_(const real_t* phi, LSMHIP_BOXP2(g), unsigned char* nb, LSMHIP_BOXP2(nbb),
+                                             int* ix, int* iy, const int* nlo, const int* nhi, int* n_lo, int* n_hi,
+                                             int* outer, const int* nlo_outer, const int* nhi_outer,
+                                             int* nlo_outer_plus, int* nhi_outer_plus, int* nlo_outer_minus,
+                                             int* nhi_outer_minus, const real_t* width, const real_t* width_inner,
+                                             const int* level) {
+  const char* who = "lsm2ddeterminenarrowband_";
+  const Box B = LSMHIP_BOX2(g), nbb = LSMHIP_BOX2(nbb);
+  if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
+  n_lo[0] = *nlo;
+  *nlo_outer_minus = *nlo_outer;
+  *nhi_outer_plus = *nhi_outer;
+  Level0 a{phi, nullptr, B.volume(), B.nx(), B.ny(), B, *width, *width_inner, 0};
+  Level0Out o{ix, iy, nullptr, *nlo, outer, *nlo_outer, *nhi_outer, nb, nbb};
+  long long tot[3] = {0, 0, 0};
+  if (!B.empty()) {
+    if (!level0(who, a, o, tot)) return;
+    if (tot[0] > (long long)*nhi - *nlo + 1 || tot[1] + tot[2] > (long long)*nhi_outer - *nlo_outer + 1) {
+      set_error(kErrArg, "%s: %lld band points (%lld outer) exceed the index arrays", who, tot[0], tot[1] + tot[2]);
+      return;
+    }
+    if (!level0_write(who, a, o)) return;
+  }
+  n_hi[0] = *nlo + (int)tot[0] - 1;
+  *nhi_outer_minus = *nlo_outer + (int)tot[1] - 1;
+  *nlo_outer_plus = *nhi_outer - (int)tot[2] + 1;
+  if (*level > 0 && n_hi[0] > n_lo[0]) grow_levels(who, nb, nbb, ix, iy, nullptr, *nlo, *nhi, n_lo, n_hi, *level);
+  finish(who);
+  take_device_flag(who);
+}
+
+// lsm2dDetermineNarrowBandFromMask, :539-615: level 0 is {mask >= 0} when
+// use_mask_sign > 0, else {mask < 0} (strict; the 3D routine uses <= 0).
+LSMHIP_EXPORT void lsm2ddeterminenarrowbandfrommask_(const real_t* mask, LSMHIP_BOXP2(mg), unsigned char* nb,
+                                                     LSMHIP_BOXP2(nbb), int* ix, int* iy, const int* nlo,
+                                                     const int* nhi, int* n_lo, int* n_hi, const int* level,
+                                                     const int* use_mask_sign) {
+  const char* who = "lsm2ddeterminenarrowbandfrommask_";
+  const Box B = LSMHIP_BOX2(mg), nbb = LSMHIP_BOX2(nbb);
+  if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
+  n_lo[0] = *nlo;
+  Level0 a{nullptr, mask, B.volume(), B.nx(), B.ny(), B, real_t(0), real_t(0), *use_mask_sign > 0 ? 1 : 3};
+  Level0Out o{ix, iy, nullptr, *nlo, nullptr, 0, 0, nb, nbb};
+  long long tot[3] = {0, 0, 0};
+  if (!B.empty()) {
+    if (!level0(who, a, o, tot)) return;
+    if (tot[0] > (long long)*nhi - *nlo + 1) {
+      set_error(kErrArg, "%s: %lld band points exceed the index arrays", who, tot[0]);
+      return;
+    }
+    if (!level0_write(who, a, o)) return;
+  }
+  n_hi[0] = *nlo + (int)tot[0] - 1;
+  grow_levels(who, nb, nbb, ix, iy, nullptr, *nlo, *nhi, n_lo, n_hi, *level);
+  finish(who);
+  take_device_flag(who);
+}
+
+// :486-533 -- the four fill-box lines, each spanning the ghost box.
+LSMHIP_EXPORT void lsm2dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb),
+                                                     const unsigned char* mark) {
+  mark_boundary("lsm2dmarknarrowbandboundarylayer_", nb, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *mark, 2);
+}
+
+// :693-772
+LSMHIP_EXPORT void lsm2dcheckouternarrowbandlayer_(int* change_sign, const real_t* phi, LSMHIP_BOXP2(g), const int* ix,
+                                                   const int* iy, const int* nlo, const int* nhi, const int* outer,
+                                                   const int* nlo_outer, const int* nhi_outer, const int* nlo_plus,
+                                                   const int* nhi_plus, const int* nlo_minus, const int* nhi_minus) {
+  check_outer("lsm2dcheckouternarrowbandlayer_", change_sign, phi, LSMHIP_BOX2(g), ix, iy, nullptr, nlo, nhi, outer,
+              nlo_outer, nhi_outer, nlo_plus, nhi_plus, nlo_minus, nhi_minus);
 }

@@ -20,117 +20,12 @@
 // point that would leave it is skipped and raises the device flag
 // (lsmhip_get_last_error then reports kErrIndex) where the reference would
 // read or write out of bounds.
-#include "lsmhip_core.hpp"
-#include "reduce.hpp"
+#include "local.hpp"
 #include "stencil.hpp"
 
 using namespace lsmhip;
 
 namespace {
-
-constexpr int kT = 256;
-
-struct IndexList {
-  const int *ix, *iy, *iz;
-  int lo, hi, base;
-};
-
-struct Mask {
-  const unsigned char* p;
-  Box b;
-  int sy, sz;
-  signed char mark;
-  int filter;  // 0: no mark filter
-  __device__ __forceinline__ signed char at(int i, int j, int k) const {
-    return (signed char)p[(i - b.ilo) + sy * (j - b.jlo) + sz * (k - b.klo)];
-  }
-};
-
-Mask make_mask(const unsigned char* nb, const Box& b, const unsigned char* mark, int filter = 1) {
-  return Mask{nb, b, b.nx(), b.nx() * b.ny(), mark ? (signed char)*mark : (signed char)0, filter};
-}
-Mask no_mask() { return Mask{nullptr, Box{0, -1, 0, -1, 0, -1}, 0, 0, 0, 0}; }
-
-// Checked array view: element access plus ghost-box membership tests.
-template <class T>
-struct CV {
-  View<T> v;
-  Box b;
-  __device__ __forceinline__ T& operator()(int i, int j, int k) const { return v(i, j, k); }
-  __device__ __forceinline__ bool has(int i, int j, int k) const { return b.has(i, j, k); }
-  // (i,j,k) with its +-w neighbours along each axis (the corners of the star's box)
-  __device__ __forceinline__ bool star(int i, int j, int k, int w) const {
-    return b.has(i - w, j - w, k - w) && b.has(i + w, j + w, k + w);
-  }
-};
-template <class T>
-CV<T> cv(T* p, const Box& b) {
-  return CV<T>{view(p, b), b};
-}
-
-// Ops return false when an access would leave an array's ghost box.
-template <class Op>
-__global__ void __launch_bounds__(kT) points_kernel(IndexList L, Mask M, Op op, int* flag) {
-  const int l = L.lo + (int)(blockIdx.x * kT + threadIdx.x);
-  if (l > L.hi) return;
-  const int i = L.ix[l - L.base], j = L.iy[l - L.base], k = L.iz[l - L.base];
-  if (M.filter) {
-    if (!M.b.has(i, j, k)) {
-      *flag = 1;
-      return;
-    }
-    if (M.at(i, j, k) > M.mark) return;
-  }
-  if (!op(i, j, k)) *flag = 1;
-}
-
-template <class Op>
-void launch_points(const char* who, const IndexList& L, const Mask& M, const Op& op) {
-  if (L.hi < L.lo) return;
-  int* flag = device_flag();
-  if (!flag) return;
-  const long long n = (long long)L.hi - L.lo + 1;
-  hipLaunchKernelGGL(points_kernel<Op>, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, stream(), L, M, op, flag);
-  check_hip(hipGetLastError(), who);
-}
-
-void finish_local(const char* who) {
-  finish(who);
-  if (synchronous()) take_device_flag(who);
-}
-
-IndexList list(const int* ix, const int* iy, const int* iz, int lo, int hi, int base) {
-  return IndexList{ix, iy, iz, lo, hi, base};
-}
-
-// ----- undivided differences over index levels (lsm3dComputeDnLOCAL, :43-138) ------------------
-struct DnOp {
-  CV<real_t> dn;
-  CV<const real_t> dm;
-  Mask m;
-  signed char markD;
-  int di, dj, dk;
-  real_t sgn;
-  __device__ bool operator()(int i, int j, int k) const {
-    if (!dn.has(i, j, k) || !m.b.has(i, j, k)) return false;
-    if (m.at(i, j, k) <= markD) {
-      if (!dm.has(i, j, k) || !dm.has(i - di, j - dj, k - dk)) return false;
-      dn(i, j, k) = sgn * (dm(i, j, k) - dm(i - di, j - dj, k - dk));
-    } else {
-      dn(i, j, k) = kBig;
-    }
-    return true;
-  }
-};
-
-// n odd: D(i) = Dm(i) - Dm(i-1); n even: D(i) = -(Dm(i) - Dm(i+1)) along axis d.
-void dn_local(const char* who, real_t* dn, const Box& dnb, const real_t* dm, const Box& dmb, int n, int d,
-              const IndexList& L, const Mask& nbm, signed char markD) {
-  const int off = n % 2 == 1 ? 1 : -1;
-  DnOp op{cv(dn, dnb), cv(dm, dmb), nbm, markD, d == 0 ? off : 0, d == 1 ? off : 0, d == 2 ? off : 0,
-          n % 2 == 1 ? real_t(1) : real_t(-1)};
-  launch_points(who, L, no_mask(), op);
-}
 
 struct Axis3 {
   CV<real_t> o[3];
@@ -403,99 +298,6 @@ struct CutOffOp {
     return true;
   }
 };
-
-// ----- reductions over index lists ----------------------------------------------------------------
-// Max over admissible points of non-negative values.  The accumulator holds
-// (IEEE bits of the value) + 1, so 0 means "no admissible point" and bit order
-// is numeric order for the non-negative values reduced here.
-template <class Op>
-__global__ void __launch_bounds__(kT) points_max_kernel(IndexList L, Mask M, Op op, unsigned long long* acc, int* flag) {
-  const int l = L.lo + (int)(blockIdx.x * kT + threadIdx.x);
-  double v = -1.0;  // below every admissible value
-  if (l <= L.hi) {
-    const int i = L.ix[l - L.base], j = L.iy[l - L.base], k = L.iz[l - L.base];
-    if (!M.b.has(i, j, k)) *flag = 1;
-    else if (M.at(i, j, k) <= M.mark && !op(i, j, k, v)) *flag = 1;
-  }
-  v = reduce::wave_max(v);
-  if ((threadIdx.x & 63) == 0 && v >= 0.0) atomicMax(acc, (unsigned long long)__double_as_longlong(v) + 1ull);
-}
-
-template <class Op>
-bool points_max(const char* who, const IndexList& L, const Mask& M, const Op& op, double* result, bool* any) {
-  *any = false;
-  if (L.hi < L.lo) return true;
-  unsigned long long* acc = (unsigned long long*)workspace(sizeof(unsigned long long));
-  int* flag = device_flag();
-  if (!acc || !flag) return false;
-  if (!check_hip(hipMemsetAsync(acc, 0, sizeof(unsigned long long), stream()), who)) return false;
-  const long long n = (long long)L.hi - L.lo + 1;
-  hipLaunchKernelGGL(points_max_kernel<Op>, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, stream(), L, M, op, acc,
-                     flag);
-  if (!check_hip(hipGetLastError(), who)) return false;
-  unsigned long long h = 0;
-  if (!check_hip(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, stream()), who)) return false;
-  if (!check_hip(hipStreamSynchronize(stream()), who)) return false;
-  take_device_flag(who);
-  if (h == 0) return true;
-  *any = true;
-  h -= 1;
-  double d;
-  memcpy(&d, &h, sizeof(d));
-  *result = d;
-  return true;
-}
-
-// Compensated FP64 sum and count over admissible points (one partial per block,
-// combined in a fixed order by reduce::sum_finish_kernel).
-template <class Op>
-__global__ void __launch_bounds__(kT) points_sum_kernel(IndexList L, Mask M, Op op, double* partials, int* counts,
-                                                        int* flag) {
-  const int l = L.lo + (int)(blockIdx.x * kT + threadIdx.x);
-  reduce::Pair p{0.0, 0.0};
-  int c = 0;
-  if (l <= L.hi) {
-    const int i = L.ix[l - L.base], j = L.iy[l - L.base], k = L.iz[l - L.base];
-    double v = 0.0;
-    if (!M.b.has(i, j, k)) *flag = 1;
-    else if (M.at(i, j, k) <= M.mark) {
-      if (op(i, j, k, v)) {
-        p = reduce::add(p, v);
-        c = 1;
-      } else {
-        *flag = 1;
-      }
-    }
-  }
-  p = reduce::wave_sum(p);
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  __shared__ double ps[kT / 64], pc[kT / 64];
-  __shared__ int cs[kT / 64];
-  if ((threadIdx.x & 63) == 0) {
-    ps[threadIdx.x >> 6] = p.s;
-    pc[threadIdx.x >> 6] = p.c;
-    cs[threadIdx.x >> 6] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    reduce::Pair q{ps[0], pc[0]};
-    int cc = cs[0];
-    for (int w = 1; w < kT / 64; w++) {
-      q = reduce::add(q, reduce::Pair{ps[w], pc[w]});
-      cc += cs[w];
-    }
-    partials[2 * blockIdx.x] = q.s;
-    partials[2 * blockIdx.x + 1] = q.c;
-    counts[blockIdx.x] = cc;
-  }
-}
-
-__global__ void count_finish_kernel(const int* counts, int n, long long* out) {
-  long long c = 0;
-  for (int t = threadIdx.x; t < n; t += 64) c += counts[t];
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if (threadIdx.x == 0) *out = c;
-}
 
 }  // namespace
 
@@ -930,30 +732,6 @@ struct GradNormOp {
 // reference sums serially; here the sum is FP64 compensated in a fixed order.
 LSMHIP_EXPORT void lsm3dcomputeavegradphilocal_(real_t* ave, const real_t* phi, LSMHIP_BOXP(pg), const real_t* dx,
                                                 const real_t* dy, const real_t* dz, LOCAL_TAIL) {
-  const char* who = "lsm3dcomputeavegradphilocal_";
-  *ave = real_t(0);
-  const IndexList L = LIST;
-  if (L.hi < L.lo) return;
-  const long long n = (long long)L.hi - L.lo + 1;
-  const int nb_ = (int)((n + kT - 1) / kT);
-  char* ws = (char*)workspace(sizeof(double) * (2 * (size_t)nb_ + 2) + sizeof(int) * (size_t)nb_ + 16);
-  int* flag = device_flag();
-  if (!ws || !flag) return;
-  double* partials = (double*)ws;
-  double* total = partials + 2 * nb_;
-  long long* count = (long long*)(total + 1);
-  int* counts = (int*)(count + 1);
-  GradNormOp op{cv(phi, LSMHIP_BOX(pg)), real_t(0.5) / *dx, real_t(0.5) / *dy, real_t(0.5) / *dz};
-  hipLaunchKernelGGL(points_sum_kernel<GradNormOp>, dim3(nb_), dim3(kT), 0, stream(), L, FBMASK, op, partials, counts,
-                     flag);
-  hipLaunchKernelGGL(reduce::sum_finish_kernel, dim3(1), dim3(reduce::kThreads), 0, stream(), partials, nb_, total);
-  hipLaunchKernelGGL(count_finish_kernel, dim3(1), dim3(64), 0, stream(), counts, nb_, count);
-  if (!check_hip(hipGetLastError(), who)) return;
-  double s = 0;
-  long long c = 0;
-  if (!check_hip(hipMemcpyAsync(&s, total, sizeof(s), hipMemcpyDeviceToHost, stream()), who)) return;
-  if (!check_hip(hipMemcpyAsync(&c, count, sizeof(c), hipMemcpyDeviceToHost, stream()), who)) return;
-  if (!check_hip(hipStreamSynchronize(stream()), who)) return;
-  take_device_flag(who);
-  *ave = c > 0 ? real_t(s / (double)c) : real_t(0);
+  points_mean("lsm3dcomputeavegradphilocal_", LIST, FBMASK,
+              GradNormOp{cv(phi, LSMHIP_BOX(pg)), real_t(0.5) / *dx, real_t(0.5) / *dy, real_t(0.5) / *dz}, ave);
 }

@@ -11,9 +11,11 @@ namespace lsmhip {
 namespace stencil {
 
 // One HJ-WENO5 reconstruction from five scaled differences, in the order of
-// src/toolbox/lsm_spatial_derivatives3d.f.in:1296-1323.
+// src/toolbox/lsm_spatial_derivatives3d.f.in:1296-1323.  tiny is the additive
+// term of eps: the templated 1e-36, or the literal 1e-35 of the 2D narrow-band
+// routine (lsm_spatial_derivatives2d_local.f.in:892).
 template <class T>
-__device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
+__device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5, T tiny = T(1.e-36)) {
   const T one_third = T(1.0 / 3.0), seven_sixths = T(7.0 / 6.0), eleven_sixths = T(11.0 / 6.0);
   const T one_sixth = T(1.0 / 6.0), five_sixths = T(5.0 / 6.0);
   const T thirteen_twelfths = T(13.0 / 12.0), one_fourth = T(0.25);
@@ -22,7 +24,7 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
   m = v3 * v3 > m ? v3 * v3 : m;
   m = v4 * v4 > m ? v4 * v4 : m;
   m = v5 * v5 > m ? v5 * v5 : m;
-  const T eps = T(1e-6) * m + T(1.e-36);
+  const T eps = T(1e-6) * m + tiny;
   const T p1 = one_third * v1 - seven_sixths * v2 + eleven_sixths * v3;
   const T p2 = -(one_sixth * v2) + five_sixths * v3 + one_third * v4;
   const T p3 = one_third * v3 + five_sixths * v4 - one_sixth * v5;

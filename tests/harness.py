@@ -254,7 +254,7 @@ def band_case(impl=None, seed=0, n=(24, 22, 20), level=4, h=None, width=None, wi
     c.arrays["n_hi"] = np.full(level + 1, -5, np.int32)
     c.boxes.update(nb=gb, gb=gb)
     c.scalars.update(nlo=0, nhi=npts - 1, nlo_outer=0, nhi_outer=npts - 1, level=level,
-                     width=width or 6 * h, width_inner=width_inner or 3 * h)
+                     width=width or 6 * h, width_inner=width_inner or 3 * h, beta=2 * h, gamma=4 * h)
     arrs, outs = run(impl, "lsm3ddeterminenarrowband_", c, device=device)
     for k in ("nb", "ix", "iy", "iz", "outer", "n_lo", "n_hi"):
         c.arrays[k] = arrs[k]
@@ -269,6 +269,58 @@ def band_case(impl=None, seed=0, n=(24, 22, 20), level=4, h=None, width=None, wi
         c.arrays["nb"] = arrs["nb"]
     c.boxes["fb"] = fb
     c.scalars.update(mark_fb=mark_fb, mark_D1=mark_d1, mark_D2=mark_d2)
+    nl, nh = c.arrays["n_lo"], c.arrays["n_hi"]
+    for L in range(level + 1):
+        c.scalars["nlo%d" % L] = int(nl[L])
+        c.scalars["nhi%d" % L] = int(nh[L])
+    c.scalars["nlo"], c.scalars["nhi"] = int(nl[0]), int(nh[0])
+    return c
+
+
+def circle_phi(gb, centre, radius, h):
+    ilo, ihi, jlo, jhi = gb
+    j, i = np.meshgrid(np.arange(jlo, jhi + 1), np.arange(ilo, ihi + 1), indexing="ij")
+    x, y = (i + 0.5) * h, (j + 0.5) * h
+    return np.sqrt((x - centre[0]) ** 2 + (y - centre[1]) ** 2) - radius
+
+
+def band_case2d(impl=None, seed=0, n=(40, 36), level=4, h=None, width=None, width_inner=None,
+                marks=(124, 125, 126, 127), device=None):
+    """The 2D counterpart of band_case: a band around two circles, one cut by
+    the domain boundary, built with ``impl`` (the oracle by default), with the
+    boundary lines of the D2, D1 and ghost boxes marked in that order."""
+    impl = impl or oracle()
+    nx, ny = n
+    h = h or 1.0 / nx
+    gb = (-3, nx + 2, -3, ny + 2)
+    fb = (0, nx - 1, 0, ny - 1)
+    c = Case(gb, fb, seed)
+    c.scalars.update(dx=h, dy=h)
+    phi = np.minimum(circle_phi(gb, (0.45, 0.5 * ny * h), 0.25, h), circle_phi(gb, (0.05, 0.1), 0.2, h))
+    phi += 1e-3 * h * c.rng.standard_normal(phi.shape)
+    c.arrays["phi"] = phi
+    npts = phi.size
+    c.arrays["nb"] = np.zeros(phi.shape, np.uint8)
+    for nm in ("ix", "iy", "outer"):
+        c.arrays[nm] = np.full(npts, -7, np.int32)
+    c.arrays["n_lo"] = np.full(level + 1, -5, np.int32)
+    c.arrays["n_hi"] = np.full(level + 1, -5, np.int32)
+    c.boxes.update(nb=gb, gb=gb)
+    c.scalars.update(nlo=0, nhi=npts - 1, nlo_outer=0, nhi_outer=npts - 1, level=level,
+                     width=width or 6 * h, width_inner=width_inner or 3 * h, beta=2 * h, gamma=4 * h)
+    arrs, outs = run(impl, "lsm2ddeterminenarrowband_", c, device=device)
+    for k in ("nb", "ix", "iy", "outer", "n_lo", "n_hi"):
+        c.arrays[k] = arrs[k]
+    (c.scalars["nlo_outer_plus"], c.scalars["nhi_outer_plus"], c.scalars["nlo_outer_minus"],
+     c.scalars["nhi_outer_minus"]) = outs
+    mark_fb, mark_d2, mark_d1, mark_gb = marks
+    for box, mark in (((-1, nx, -1, ny), mark_d2), ((-2, nx + 1, -2, ny + 1), mark_d1), (gb, mark_gb)):
+        c.boxes["fb"] = box
+        c.scalars["mark"] = mark
+        arrs, _ = run(impl, "lsm2dmarknarrowbandboundarylayer_", c, device=device)
+        c.arrays["nb"] = arrs["nb"]
+    c.boxes["fb"] = fb
+    c.scalars.update(mark_fb=mark_fb, mark_D1=mark_d1, mark_D2=mark_d2, mark_D3=mark_d1)
     nl, nh = c.arrays["n_lo"], c.arrays["n_hi"]
     for L in range(level + 1):
         c.scalars["nlo%d" % L] = int(nl[L])

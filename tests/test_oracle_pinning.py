@@ -445,7 +445,10 @@ def _full_band_case():
     return c
 
 
-TWINS = [s for s in capi.SPECS if s.endswith("local_") and s.replace("local_", "_") in capi.SPECS]
+TWINS = [s for s in capi.SPECS
+         if s.startswith("lsm3d") and s.endswith("local_") and s.replace("local_", "_") in capi.SPECS]
+TWINS_2D = [s for s in capi.SPECS
+            if s.startswith("lsm2d") and s.endswith("local_") and s.replace("local_", "_") in capi.SPECS]
 
 
 @pytest.mark.parametrize("sym", TWINS)
@@ -474,6 +477,46 @@ def test_local_equals_whole_grid_on_full_band(sym):
     for name, a in aw.items():
         if name in al and a.dtype.kind == "f":
             assert np.array_equal(al[name][inner][keep], a[inner][keep]), name
+            compared += 1
+    assert compared or ol is not None
+
+
+def _full_band_case2d():
+    """The 2D counterpart of _full_band_case: every ghost-box cell at level 0,
+    boundary lines marked as usual, dy != dx."""
+    c = H.band_case2d(level=1, width=100.0, width_inner=50.0)
+    c.arrays["mask"] = c.rng.standard_normal(c.arrays["phi"].shape)
+    c.boxes["mask"] = c.gb
+    c.scalars["dy"] = 1.3 * c.scalars["dx"]
+    n0 = c.scalars["nhi0"]
+    assert c.scalars["nlo0"] == 0 and n0 + 1 == c.arrays["phi"].size
+    for L in range(1, 4):
+        c.scalars["nlo%d" % L], c.scalars["nhi%d" % L] = n0 + 1, n0
+    return c
+
+
+@pytest.mark.parametrize("sym", TWINS_2D)
+def test_local_2d_equals_whole_grid_on_full_band(sym):
+    """The 2D narrow-band routines on a band covering the grid compute the 2D
+    whole-grid routines on the fill box, bitwise (scalars too).  HJ-WENO5 and
+    HJ-ENO3 read differences three cells out, where the outermost ghost line
+    holds the 'big' value, so they are compared three cells inside; the
+    orders-4 curvature reads phi_x two rows away and takes the ghost box as
+    its gradient box."""
+    c = _full_band_case2d()
+    if sym == "lsm2dcomputemeancurvatureorder4local_":
+        c.boxes["g"] = c.gb
+    o = O()
+    al, ol = H.run(o, sym, c)
+    aw, ow = H.run(o, sym.replace("local_", "_"), c)
+    assert ol == ow
+    inner = (slice(3, -3),) * 2
+    if sym in ("lsm2dhjweno5local_", "lsm2dhjeno3local_"):
+        inner = (slice(6, -6),) * 2
+    compared = 0
+    for name, a in aw.items():
+        if name in al and a.dtype.kind == "f" and name not in ("D1", "D2", "D3"):
+            assert np.array_equal(al[name][inner], a[inner]), name
             compared += 1
     assert compared or ol is not None
 
