@@ -248,6 +248,51 @@ def kernel_table(n, reps, warmup):
     return rows
 
 
+def band_table(n, steps, warmup):
+    """The narrow-band workload (64 spheres at n^3, band 6dx / 3dx, levels
+    0..3, local WENO5 + constant normal speed + cut-off + local TVD-RK3, as
+    lsmlib_amd.solver.NarrowBandRun) through the drop-in entry points, FP64 and
+    FP32: time of one band construction (level 0 compaction, levels 1..3,
+    boundary layers) and of a step (CFL, three stages, ghost fills), band
+    sizes, and band cell-updates/s = level-0 points x steps / time; the step
+    through the drop-in local chain and through the fused band stages."""
+    import torch
+    from lsmlib_amd.solver import Grid, NarrowBandRun, phi_spheres_min
+    g = Grid((n, n, n))
+    phi0, _ = phi_spheres_min(g, 3, 64, (0.1, 0.9), (0.03, 0.1))
+    rows = []
+    for prec, fused in (("f64", False), ("f32", False), ("f64", True), ("f32", True)):
+        run = NarrowBandRun(n, precision=prec, steps=steps, rebuild_every=1 << 30, phi0=phi0, fused=fused)
+        _sync()
+        t0 = time.perf_counter()
+        run.rebuild()
+        _sync()
+        t_band = time.perf_counter() - t0
+        run.nsteps = 1  # keep this band: step() rebuilds only when nsteps % rebuild_every == 0
+        for _ in range(warmup):
+            run.step()
+        _sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run.step()
+        _sync()
+        el = time.perf_counter() - t0
+        run.solver.lib.check()
+        lv0 = int(run.n_hi[0] - run.n_lo[0] + 1)
+        lv3 = int(run.n_hi[run.level] - run.n_lo[0] + 1)
+        row = {"workload": "narrow band %d^3, 64 spheres, levels 0..%d" % (n, run.level), "dtype": prec,
+               "path": "fused band stages" if fused else "drop-in local chain",
+               "band_level0_points": lv0, "band_points_levels_0_3": lv3, "band_build_ms": round(1e3 * t_band, 3),
+               "ms_per_step": round(1e3 * el / steps, 3), "band_cell_updates_per_s": round(lv0 * steps / el, 1)}
+        rows.append(row)
+        print("[bench] band %s %s: build %.2f ms, %.3f ms/step, %d level-0 points, %.3e band cell-updates/s" % (
+            prec, row["path"], row["band_build_ms"], row["ms_per_step"], lv0, row["band_cell_updates_per_s"]),
+            file=sys.stderr, flush=True)
+        del run
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +306,9 @@ def main():
     ap.add_argument("--kernels", action="store_true",
                     help="time the normal / curvature / orthogonalization / field-extension entry points one by "
                          "one at n^3 (--steps launches each) instead of the level-set step")
+    ap.add_argument("--band", action="store_true",
+                    help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
+                         "instead of the whole-grid step")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device copy of 1 GiB before timing (the known-size kernel for HBM counters)")
     a = ap.parse_args()
@@ -277,6 +325,11 @@ def main():
         print(json.dumps({"metric": "drop-in entry point HBM roofline @%d^3" % a.n, "unit": "GB/s",
                           "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                           "data": "synthetic", "kernels": rows}))
+        return
+    if a.band:
+        rows = band_table(a.n, a.steps, a.warmup)
+        print(json.dumps({"metric": "narrow-band local step @%d^3" % a.n, "unit": "band cell-updates/s",
+                          "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows}))
         return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

@@ -90,6 +90,42 @@ int lsmhip_deform_velocity_fill(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_R
 /* k planes a fused-stage block marches through (default 64). */
 void lsmhip_set_fused_kchunk(int planes);
 
+/* ---- fused narrow-band stages: one TVD-RK stage of normal motion on a band ----
+ * The band as the reference's local driver builds it (LSM3D_DETERMINE_NARROW_BAND
+ * with at least three levels, boundary layers marked): index_x/y/z addressed
+ * from element nlo_index0, level-0 entries nlo_index0..nhi_index0, narrow_band
+ * on nb_gb, phi_stage / phi_n / vn_field / phi_out on gb.  One thread per
+ * level-0 point with mark <= mark_fb; the differences of the WENO5 stencil
+ * take the 'big' value where a neighbour's mark exceeds mark_D1, as
+ * HJ_WENO5_LOCAL stores them.  A point whose stencil reaches a cell outside
+ * the band (mark 0) is skipped and reported (LSMHIP_ERR_INDEX).  Returns 0 or -1. */
+
+/* HJ_WENO5_LOCAL (lsm3dhjweno5local_, levels 0..3) -> ZERO_OUT_LOCAL ->
+ * ADD_[CONST_]NORMAL_VEL_TERM_LOCAL -> MULTIPLY_CUT_OFF_LSE_RHS_LOCAL(phi_stage,
+ * beta, gamma; none when gamma <= 0) -> TVD_RK3_STAGEn_LOCAL (stage 1, 2, 3; 22 =
+ * TVD-RK2 stage 2).  FP64: bitwise equal to that chain; phi_n may be NULL for stage 1.
+ * nhi_copy >= nlo_index0 folds in COPY_DATA_LOCAL(phi_out, phi_n) over entries
+ * nlo_index0..nhi_copy (e.g. levels 0..3) before the stage: the entries the
+ * stage does not update (beyond nhi_index0, or marked above mark_fb) take
+ * phi_n (phi_stage at stage 1); nhi_copy < nlo_index0 copies nothing. */
+int lsmhip_weno5_local_stage_normal(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                                    LSMHIP_BOX_ARGS(gb), int stage, LSMLIB_REAL vn, const LSMLIB_REAL *vn_field,
+                                    LSMLIB_REAL dt, LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                    const int *index_x, const int *index_y, const int *index_z, int nlo_index0,
+                                    int nhi_index0, int nhi_copy, const unsigned char *narrow_band,
+                                    LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb, unsigned char mark_D1,
+                                    LSMLIB_REAL beta, LSMLIB_REAL gamma);
+/* HJ_WENO5_LOCAL -> COMPUTE_STABLE_[CONST_]NORMAL_VEL_DT_LOCAL: dt = cfl / (max
+ * over level-0 points of |V| / |grad phi|_max+ * (phi_x/dx + ...) + tiny) with the
+ * whole-grid formula (no |grad phi| filter).  FP64: bitwise equal to that chain
+ * for constant vn. */
+int lsmhip_weno5_local_normal_dt(LSMLIB_REAL *dt, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb), LSMLIB_REAL vn,
+                                 const LSMLIB_REAL *vn_field, LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                 LSMLIB_REAL cfl_number, const int *index_x, const int *index_y,
+                                 const int *index_z, int nlo_index0, int nhi_index0,
+                                 const unsigned char *narrow_band, LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb,
+                                 unsigned char mark_D1);
+
 /* ---- reverse mode of the normal-motion stage (for gradient-based inverse problems) ----
  * Reverse of lsmhip_weno5_stage_normal for stage 1, 2, 3 (TVD-RK3) or 22
  * (TVD-RK2 stage 2): with out = a u_c + b (u_s + dt L(u_s)), L = -V sqrt(G),

@@ -1,0 +1,299 @@
+// fused_local3d.hip -- one TVD Runge-Kutta stage of normal motion on a narrow
+// band in a single kernel, and the matching CFL step.
+//
+// The drop-in chain for a band stage runs HJ_WENO5_LOCAL (per axis: first
+// differences over levels 0..3 into D1, then the plus/minus reconstruction at
+// level 0), ZERO_OUT_LOCAL, ADD_[CONST_]NORMAL_VEL_TERM_LOCAL,
+// MULTIPLY_CUT_OFF_LSE_RHS_LOCAL and TVD_RK3_STAGEn_LOCAL: nine launches that
+// write and re-read six derivative arrays, D1 and the RHS.  Here one thread
+// per level-0 point reads the 19-point cross of phi and the marks of the 18
+// neighbours, forms the differences the drop-in would have stored (the 'big'
+// value where a mark exceeds mark_D1), and goes straight to the RK update.
+//
+// Equality with the drop-in chain: the differences the chain reads at a
+// level-0 point lie within three cells along an axis, and a cell that close to
+// a level-0 point is in levels 0..3 of any band grown to at least three levels
+// (first-claim growth reaches every cell within distance d by level d), so the
+// chain has written them in this call.  A neighbour outside the band (mark 0)
+// would be a stale difference in the chain; the kernel skips such a point and
+// reports it (LSMHIP_ERR_INDEX) instead.  With that, FP64 results are bitwise
+// the chain's; FP32 uses the rearranged WENO5 of the whole-grid fused stages.
+#include "local.hpp"
+#include "stencil.hpp"
+
+using namespace lsmhip;
+
+namespace {
+
+struct LocalStage {
+  IndexList L;        // level-0 points
+  Mask fbm;           // narrow_band, mark_fb
+  signed char markD1;
+  const real_t *us, *uc, *vnf;
+  real_t* out;
+  Box gb;             // us, uc, vnf and out share it
+  int sy, sz;
+  real_t inv[3];
+  real_t vn, dt;
+  int cutoff;
+  real_t beta, gamma, c1, c2;
+  int ncopy;          // entries lo..lo+ncopy-1 not updated by the stage take phi_n (0: none)
+};
+
+// The 19-point cross of phi around (i, j, k) (offset p) and the marks of the
+// 18 neighbours whose differences the stencil reads, all loads issued before
+// any is tested.  Returns false (and flags) when the cross leaves gb or the
+// band's box, or reaches a cell outside the band.
+__device__ __forceinline__ bool gather_cross(const LocalStage& a, int i, int j, int k, long long p, real_t f[3][7],
+                                             signed char m[3][6], int* flag) {
+  const Box& nbb = a.fbm.b;
+  if (!a.gb.has(i - 3, j - 3, k - 3) || !a.gb.has(i + 3, j + 3, k + 3) || !nbb.has(i - 2, j - 2, k - 2) ||
+      !nbb.has(i + 3, j + 3, k + 3)) {
+    *flag = 1;
+    return false;
+  }
+  const long long st[3] = {1, a.sy, a.sz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int s = -3; s <= 3; s++) f[d][s + 3] = a.us[p + s * st[d]];
+#pragma unroll
+    for (int s = -2; s <= 3; s++)
+      m[d][s + 2] = a.fbm.at(i + (d == 0 ? s : 0), j + (d == 1 ? s : 0), k + (d == 2 ? s : 0));
+  }
+  bool out = false;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int s = 0; s < 6; s++) out |= m[d][s] == 0;
+  if (out) *flag = 1;
+  return !out;
+}
+
+// Plus and minus WENO5 derivatives along axis d from the gathered cross, with
+// the differences HJ_WENO5_LOCAL stores (the 'big' value where a mark exceeds
+// mark_D1).
+__device__ __forceinline__ void plus_minus_local(const LocalStage& a, const real_t f[7], const signed char m[6],
+                                                 int d, real_t& plus, real_t& minus) {
+  real_t dd[6];  // dd[s+2] = D1(s), s = -2..3
+#pragma unroll
+  for (int s = -2; s <= 3; s++) dd[s + 2] = m[s + 2] <= a.markD1 ? f[s + 3] - f[s + 2] : kBig;
+  const real_t inv = a.inv[d];
+  if (sizeof(real_t) == 8) {
+    plus = stencil::weno5(dd[5] * inv, dd[4] * inv, dd[3] * inv, dd[2] * inv, dd[1] * inv);
+    minus = stencil::weno5(dd[0] * inv, dd[1] * inv, dd[2] * inv, dd[3] * inv, dd[4] * inv);
+  } else {
+    using namespace stencil::fastf;
+    plus = (real_t)rev(win5((float)dd[1], (float)dd[2], (float)dd[3], (float)dd[4], (float)dd[5]), (float)dd[2],
+                       (float)dd[3], (float)dd[4]) * inv;
+    minus = (real_t)fwd(win5((float)dd[0], (float)dd[1], (float)dd[2], (float)dd[3], (float)dd[4]), (float)dd[1],
+                        (float)dd[2], (float)dd[3]) * inv;
+  }
+}
+
+// Plus/minus derivatives on all three axes at one point; false when the
+// stencil may not be evaluated there (flagged).
+__device__ __forceinline__ bool derivatives_local(const LocalStage& a, int i, int j, int k, long long p, real_t g[6],
+                                                  int* flag) {
+  real_t f[3][7];
+  signed char m[3][6];
+  if (!gather_cross(a, i, j, k, p, f, m, flag)) return false;
+  plus_minus_local(a, f[0], m[0], 0, g[0], g[3]);
+  plus_minus_local(a, f[1], m[1], 1, g[1], g[4]);
+  plus_minus_local(a, f[2], m[2], 2, g[2], g[5]);
+  return true;
+}
+
+__device__ __forceinline__ real_t local_sqrt(real_t x) {
+  if (sizeof(real_t) == 8) return sqrt(x);
+  return (real_t)__builtin_amdgcn_sqrtf((float)x);
+}
+
+// Level-0 entry t of the list: its point, offset in gb, and whether it takes
+// part (inside the band's box, mark <= mark_fb).
+__device__ __forceinline__ bool band_point(const LocalStage& a, long long t, int& i, int& j, int& k, long long& p,
+                                           int* flag) {
+  a.L.at(a.L.lo + (int)t, i, j, k);
+  if (!a.fbm.b.has(i, j, k) || !a.gb.has(i, j, k)) {
+    *flag = 1;
+    return false;
+  }
+  if (a.fbm.at(i, j, k) > a.fbm.mark) return false;
+  p = (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
+  return true;
+}
+
+template <int STAGE>
+__device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, real_t r) {
+  if (STAGE == 1) return uc + dt * r;
+  if (STAGE == 2) return real_t(0.75) * uc + real_t(0.25) * (us + dt * r);
+  if (STAGE == 3) return real_t(1.0 / 3.0) * uc + real_t(2.0 / 3.0) * (us + dt * r);
+  return real_t(0.5) * (uc + us + dt * r);  // TVD-RK2 stage 2
+}
+
+// Entry t that the stage does not update, within the copy range: out = phi_n
+// there (phi_stage at stage 1, where it is phi_n), as COPY_DATA_LOCAL(out, phi_n)
+// over those levels before the stage would leave it.
+template <int STAGE>
+__device__ __forceinline__ void pass_through(const LocalStage& a, long long t, int* flag) {
+  if (t >= a.ncopy) return;
+  int i, j, k;
+  a.L.at(a.L.lo + (int)t, i, j, k);
+  if (!a.gb.has(i, j, k)) {
+    *flag = 1;
+    return;
+  }
+  const long long p = (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
+  a.out[p] = STAGE == 1 ? a.us[p] : a.uc[p];
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(kT) local_normal_stage_kernel(LocalStage a, int* flag) {
+  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  if (t > (long long)a.L.hi - a.L.lo) {
+    pass_through<STAGE>(a, t, flag);
+    return;
+  }
+  int i, j, k;
+  long long p;
+  if (!band_point(a, t, i, j, k, p, flag)) {
+    if (a.fbm.b.has(i, j, k)) pass_through<STAGE>(a, t, flag);  // mark above mark_fb
+    return;
+  }
+  const real_t v = a.vnf ? a.vnf[p] : a.vn;
+  const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
+  real_t g[6];  // xp, yp, zp, xm, ym, zm
+  if (!derivatives_local(a, i, j, k, p, g, flag)) return;
+  real_t r = real_t(0);
+  if (fabs(v) >= kZeroTol) r = r - v * local_sqrt(stencil::godunov_sq(v > real_t(0), g[0], g[1], g[2], g[3], g[4], g[5]));
+  const real_t us = a.us[p];
+  if (a.cutoff) {
+    const real_t ab = fabs(us);
+    real_t cut;
+    if (ab <= a.beta) {
+      cut = real_t(1);
+    } else if (ab <= a.gamma) {
+      const real_t q = ab - a.gamma;
+      cut = (q * q * (real_t(2) * ab + a.c1)) / a.c2;
+    } else {
+      cut = real_t(0);
+    }
+    r = cut * r;
+  }
+  a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, r);
+}
+
+// CFL of normal motion at one level-0 point (lsm3dComputeStableNormalVelDtLOCAL's
+// H over dx with the constant-speed form when vnf is null, no |grad phi| filter).
+struct LocalDtOp {
+  LocalStage a;
+  real_t mdx2;
+  int* flag;
+  __device__ bool operator()(int i, int j, int k, double& r) const {
+    if (!a.gb.has(i, j, k)) return false;
+    const long long p =
+        (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
+    real_t g[6];
+    if (!derivatives_local(a, i, j, k, p, g, flag)) return false;
+    const real_t av = a.vnf ? fabs(a.vnf[p]) : fabs(a.vn);
+    real_t n;
+    r = (double)stencil::normal_h_over_dx(av, g[0], g[1], g[2], g[3], g[4], g[5], mdx2, a.inv[0], a.inv[1], a.inv[2],
+                                          &n);
+    return true;
+  }
+};
+
+LocalStage make_stage(real_t* out, const real_t* us, const real_t* uc, const Box& gb, const int* ix, const int* iy,
+                      const int* iz, int nlo, int nhi, const unsigned char* nb, const Box& nbb, unsigned char mark_fb,
+                      unsigned char mark_d1, real_t dx, real_t dy, real_t dz, real_t vn, const real_t* vnf) {
+  LocalStage a{};
+  a.L = IndexList{ix, iy, iz, nlo, nhi, nlo};
+  a.fbm = make_mask(nb, nbb, &mark_fb, 1);
+  a.markD1 = (signed char)mark_d1;
+  a.us = us;
+  a.uc = uc;
+  a.vnf = vnf;
+  a.out = out;
+  a.gb = gb;
+  a.sy = gb.nx();
+  a.sz = gb.nx() * gb.ny();
+  a.inv[0] = real_t(1.0) / dx;
+  a.inv[1] = real_t(1.0) / dy;
+  a.inv[2] = real_t(1.0) / dz;
+  a.vn = vn;
+  return a;
+}
+
+}  // namespace
+
+LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LSMLIB_REAL* phi_stage,
+                                                  const LSMLIB_REAL* phi_n, LSMHIP_BOX_ARGS(gb), int stage,
+                                                  LSMLIB_REAL vn, const LSMLIB_REAL* vn_field, LSMLIB_REAL dt,
+                                                  LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz, const int* index_x,
+                                                  const int* index_y, const int* index_z, int nlo_index0,
+                                                  int nhi_index0, int nhi_copy, const unsigned char* narrow_band,
+                                                  LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb,
+                                                  unsigned char mark_D1, LSMLIB_REAL beta, LSMLIB_REAL gamma) {
+  const char* who = "lsmhip_weno5_local_stage_normal";
+  const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
+  const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
+  if (stage != 1 && stage != 2 && stage != 3 && stage != 22) {
+    set_error(kErrArg, "%s: stage %d is not 1, 2, 3 or 22", who, stage);
+    return -1;
+  }
+  if (stage != 1 && !phi_n) {
+    set_error(kErrArg, "%s: stage %d needs phi_n", who, stage);
+    return -1;
+  }
+  if (!check_size(gb, who) || !check_size(nbb, who)) return -1;
+  const int hi = nhi_copy > nhi_index0 ? nhi_copy : nhi_index0;
+  if (hi < nlo_index0) return 0;
+  LocalStage a = make_stage(phi_out, phi_stage, phi_n, gb, index_x, index_y, index_z, nlo_index0, nhi_index0,
+                            narrow_band, nbb, mark_fb, mark_D1, dx, dy, dz, vn, vn_field);
+  a.ncopy = nhi_copy >= nlo_index0 ? nhi_copy - nlo_index0 + 1 : 0;
+  a.dt = dt;
+  a.cutoff = gamma > real_t(0);
+  a.beta = beta;
+  a.gamma = gamma;
+  a.c1 = gamma - real_t(3) * beta;
+  real_t c2 = gamma - beta;
+  a.c2 = c2 * c2 * c2;
+  int* flag = device_flag();
+  if (!flag) return -1;
+  const long long n = (long long)hi - nlo_index0 + 1;
+  const dim3 grid((unsigned)((n + kT - 1) / kT));
+  switch (stage) {
+    case 1: hipLaunchKernelGGL(local_normal_stage_kernel<1>, grid, dim3(kT), 0, stream(), a, flag); break;
+    case 2: hipLaunchKernelGGL(local_normal_stage_kernel<2>, grid, dim3(kT), 0, stream(), a, flag); break;
+    case 3: hipLaunchKernelGGL(local_normal_stage_kernel<3>, grid, dim3(kT), 0, stream(), a, flag); break;
+    default: hipLaunchKernelGGL(local_normal_stage_kernel<22>, grid, dim3(kT), 0, stream(), a, flag); break;
+  }
+  if (!check_hip(hipGetLastError(), who)) return -1;
+  if (!finish(who)) return -1;
+  if (synchronous()) take_device_flag(who);
+  return lsmhip_get_last_error() == 0 ? 0 : -1;
+}
+
+LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt(LSMLIB_REAL* dt, const LSMLIB_REAL* phi, LSMHIP_BOX_ARGS(gb),
+                                               LSMLIB_REAL vn, const LSMLIB_REAL* vn_field, LSMLIB_REAL dx,
+                                               LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL cfl_number,
+                                               const int* index_x, const int* index_y, const int* index_z,
+                                               int nlo_index0, int nhi_index0, const unsigned char* narrow_band,
+                                               LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb, unsigned char mark_D1) {
+  const char* who = "lsmhip_weno5_local_normal_dt";
+  const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
+  const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
+  if (!check_size(gb, who) || !check_size(nbb, who)) return -1;
+  int* flag = device_flag();
+  if (!flag) return -1;
+  LocalStage a = make_stage(nullptr, phi, nullptr, gb, index_x, index_y, index_z, nlo_index0, nhi_index0,
+                            narrow_band, nbb, mark_fb, mark_D1, dx, dy, dz, vn, vn_field);
+  real_t h = dx > dy ? dx : dy;
+  h = h > dz ? h : dz;
+  double m = -1.0;
+  bool any = false;
+  if (!points_max(who, a.L, a.fbm, LocalDtOp{a, h * h, flag}, &m, &any)) return -1;
+  if (!any) m = -1.0;
+  *dt = cfl_number / (real_t(m) + kTiny);
+  return lsmhip_get_last_error() == 0 ? 0 : -1;
+}

@@ -133,45 +133,65 @@ void dn_local(const char* who, real_t* dn, const Box& dnb, const real_t* dm, con
 }
 
 // ----- reductions over index lists ----------------------------------------------------------------
-// Max over admissible points of non-negative values.  The accumulator holds
-// (IEEE bits of the value) + 1, so 0 means "no admissible point" and bit order
-// is numeric order for the non-negative values reduced here.
+// Max over admissible points of non-negative values (-1 stands for "no
+// value": a point that is not admissible, or an op that declines it).
 template <class Op>
-__global__ void __launch_bounds__(kT) points_max_kernel(IndexList L, Mask M, Op op, unsigned long long* acc, int* flag) {
-  const int l = L.lo + (int)(blockIdx.x * kT + threadIdx.x);
-  double v = -1.0;  // below every admissible value
-  if (l <= L.hi) {
+__global__ void __launch_bounds__(kT) points_max_kernel(IndexList L, Mask M, Op op, double* part, int* flag) {
+  // one thread per list entry, one partial per block (no atomics: a single
+  // accumulator would serialise its updates)
+  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  double m = -1.0;  // below every admissible value
+  if (t <= (long long)L.hi - L.lo) {
     int i, j, k;
-    L.at(l, i, j, k);
+    L.at(L.lo + (int)t, i, j, k);
     if (!M.b.has(i, j, k)) *flag = 1;
-    else if (M.at(i, j, k) <= M.mark && !op(i, j, k, v)) *flag = 1;
+    else if (M.at(i, j, k) <= M.mark && !op(i, j, k, m)) *flag = 1;
   }
-  v = reduce::wave_max(v);
-  if ((threadIdx.x & 63) == 0 && v >= 0.0) atomicMax(acc, (unsigned long long)__double_as_longlong(v) + 1ull);
+  m = reduce::wave_max(m);
+  __shared__ double w[kT / 64];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kT / 64; q++) m = w[q] > m ? w[q] : m;
+    part[blockIdx.x] = m;
+  }
 }
 
+constexpr int kFinish = 1024;  // threads of the one-block finishing reduction
+__global__ void __launch_bounds__(kFinish) max_finish_kernel(const double* part, long long n, double* out) {
+  double m = -1.0;
+  for (long long t = threadIdx.x; t < n; t += kFinish) m = part[t] > m ? part[t] : m;
+  m = reduce::wave_max(m);
+  __shared__ double w[kFinish / 64];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kFinish / 64; q++) m = w[q] > m ? w[q] : m;
+    *out = m;
+  }
+}
+
+// Max of op's values over admissible points: *any is false when no point is
+// admissible (or every value was below zero).
 template <class Op>
 bool points_max(const char* who, const IndexList& L, const Mask& M, const Op& op, double* result, bool* any) {
   *any = false;
   if (L.hi < L.lo) return true;
-  unsigned long long* acc = (unsigned long long*)workspace(sizeof(unsigned long long));
-  int* flag = device_flag();
-  if (!acc || !flag) return false;
-  if (!check_hip(hipMemsetAsync(acc, 0, sizeof(unsigned long long), stream()), who)) return false;
   const long long n = (long long)L.hi - L.lo + 1;
-  hipLaunchKernelGGL(points_max_kernel<Op>, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, stream(), L, M, op, acc,
-                     flag);
+  const long long nb = (n + kT - 1) / kT;
+  double* part = (double*)workspace(sizeof(double) * (size_t)(nb + 1));
+  int* flag = device_flag();
+  if (!part || !flag) return false;
+  hipLaunchKernelGGL(points_max_kernel<Op>, dim3((unsigned)nb), dim3(kT), 0, stream(), L, M, op, part, flag);
+  hipLaunchKernelGGL(max_finish_kernel, dim3(1), dim3(kFinish), 0, stream(), part, nb, part + nb);
   if (!check_hip(hipGetLastError(), who)) return false;
-  unsigned long long h = 0;
-  if (!check_hip(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, stream()), who)) return false;
+  double h = -1.0;
+  if (!check_hip(hipMemcpyAsync(&h, part + nb, sizeof(h), hipMemcpyDeviceToHost, stream()), who)) return false;
   if (!check_hip(hipStreamSynchronize(stream()), who)) return false;
   take_device_flag(who);
-  if (h == 0) return true;
+  if (!(h >= 0.0)) return true;
   *any = true;
-  h -= 1;
-  double d;
-  memcpy(&d, &h, sizeof(d));
-  *result = d;
+  *result = h;
   return true;
 }
 

@@ -557,12 +557,16 @@ class NarrowBandRun:
       three stages of: HJ_WENO5_LOCAL (stage 1 reuses the derivatives above)
         -> ZERO_OUT_LOCAL -> ADD_CONST_NORMAL_VEL_TERM_LOCAL (V = 1)
         -> MULTIPLY_CUT_OFF_LSE_RHS_LOCAL(beta = width_inner, gamma = width)
-        -> TVD_RK3_STAGEn_LOCAL -> signed linear extrapolation fill."""
+        -> TVD_RK3_STAGEn_LOCAL -> signed linear extrapolation fill.
+    fused=True runs the same step through lsmhip_weno5_local_normal_dt and one
+    lsmhip_weno5_local_stage_normal launch per stage, the copies folded into
+    stages 1 and 2 (FP64: bitwise the same fields)."""
 
     MARKS = (124, 125, 126, 127)  # fb, D2, D1, gb
     CFL = 0.5
 
-    def __init__(self, n=512, seed=3, precision="f64", steps=200, rebuild_every=10, level=3, phi0=None):
+    def __init__(self, n=512, seed=3, precision="f64", steps=200, rebuild_every=10, level=3, phi0=None,
+                 fused=False):
         import torch
         self.grid = g = Grid((n, n, n))
         self.solver = s = Solver(g, precision, bc="signed_linear")
@@ -588,6 +592,15 @@ class NarrowBandRun:
         self.dts, self.band_sizes = [], []
         self.api = s.lib.api
         self.on_rebuild = None  # callable(run) after each rebuild, e.g. to record the band
+        self.fused = fused
+        if fused:
+            L = s.lib
+            P, I, R, U8 = ctypes.c_void_p, ctypes.c_int, s.real, ctypes.c_ubyte
+            box = [ctypes.POINTER(ctypes.c_int)] * 6
+            self._fstage = L.fn("lsmhip_weno5_local_stage_normal",
+                                [P, P, P] + box + [I, R, P, R, R, R, R, P, P, P, I, I, I, P] + box + [U8, U8, R, R])
+            self._fdt = L.fn("lsmhip_weno5_local_normal_dt",
+                             [ctypes.POINTER(R), P] + box + [R, P, R, R, R, R, P, P, P, I, I, P] + box + [U8, U8])
 
     def boxes(self):
         n, gb = self.grid.n, self.grid.gb
@@ -618,6 +631,8 @@ class NarrowBandRun:
                             self.ix, self.iy, self.iz, *lv[:8], self.nb, gb, self.MARKS[0], self.MARKS[2])
 
     def step(self):
+        if self.fused:
+            return self._step_fused()
         a, g, G = self.api, self.grid, self.grads
         gb = g.gb
         if self.nsteps % self.rebuild_every == 0:
@@ -650,6 +665,32 @@ class NarrowBandRun:
                                          mfb)
             self.solver.fill_ghosts(out)
         self.solver.lib.check()
+        self.nsteps += 1
+
+    def _step_fused(self):
+        a, g, s = self.api, self.grid, self.solver
+        gb = g.gb
+        if self.nsteps % self.rebuild_every == 0:
+            self.rebuild()
+        lo0, hi0, hi3 = int(self.n_lo[0]), int(self.n_hi[0]), int(self.n_hi[self.level])
+        band = (self.ix, self.iy, self.iz)
+        ptrs = [b.data_ptr() for b in band]
+        mfb, md1 = self.MARKS[0], self.MARKS[2]
+        phi, (p1, p2) = self.phi, self.work
+        gbp = s._gb()
+        dt = s.real(0)
+        s._ok(self._fdt(ctypes.byref(dt), phi.data_ptr(), *gbp, 1.0, None, *g.dx, self.CFL, *ptrs, lo0, hi0,
+                        self.nb.data_ptr(), *gbp, mfb, md1))
+        dt = dt.value
+        self.dts.append(dt)
+        for st, (src, out) in enumerate(((phi, p1), (p1, p2), (p2, phi)), start=1):
+            # stages 1 and 2 also carry phi^n into their outputs over levels
+            # 0..3 (the drop-in sequence copies phi there before stage 1)
+            s._ok(self._fstage(out.data_ptr(), src.data_ptr(), phi.data_ptr(), *gbp, st, 1.0, None, dt, *g.dx,
+                               *ptrs, lo0, hi0, hi3 if st < 3 else -1, self.nb.data_ptr(), *gbp, mfb, md1,
+                               self.width_inner, self.width))
+            s.fill_ghosts(out)
+        s.lib.check()
         self.nsteps += 1
 
     def run(self):

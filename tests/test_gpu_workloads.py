@@ -94,6 +94,21 @@ def test_c2_periodic_full_is_bitwise_the_reference():
     assert _digest(phi) == g["sha256_phi_f64"]
 
 
+def test_c3_band_full_fused_is_bitwise_the_reference():
+    """The 512^3, 200-step band workload through the fused band stages: the
+    same bands and the reference kernels' SHA-256 of the final field."""
+    from lsmlib_amd.solver import NarrowBandRun
+    g = _golden("c3_band_512.json")
+    bands = []
+    r = NarrowBandRun(n=g["n"], seed=g["seed"], steps=g["steps"], rebuild_every=g["rebuild_every"], fused=True)
+    r.on_rebuild = _record_bands(bands)
+    phi = r.solver.host(r.run())
+    assert r.band_sizes == g["band_level0_sizes"]
+    assert bands == g["band_sha256"]
+    assert r.dts[0] == float.fromhex(g["dt_first"]) and r.dts[-1] == float.fromhex(g["dt_last"])
+    assert _digest(phi) == g["sha256_phi_f64"]
+
+
 def _record_bands(store):
     mg = _make_goldens()
 
@@ -119,6 +134,58 @@ def test_c3_band_small_is_bitwise_the_oracle():
     assert bands == bands_ref
     assert r.dts == dts_ref
     assert np.array_equal(phi, phi_ref)
+
+
+def test_c3_band_small_fused_is_bitwise_the_oracle():
+    """The same 64^3 sequence with one fused launch per stage
+    (lsmhip_weno5_local_stage_normal, lsmhip_weno5_local_normal_dt): field,
+    every dt and every band equal the oracle's drop-in chain bit for bit."""
+    import harness as H
+    from lsmlib_amd.solver import NarrowBandRun
+    phi_ref, dts_ref, bands_ref = _make_goldens().c3(n=64, steps=12, rebuild_every=5, impl=H.oracle(),
+                                                     want_field=True)
+    bands = []
+    r = NarrowBandRun(n=64, steps=12, rebuild_every=5, fused=True)
+    r.on_rebuild = _record_bands(bands)
+    phi = r.solver.host(r.run())
+    assert bands == bands_ref
+    assert r.dts == dts_ref
+    assert np.array_equal(phi, phi_ref)
+
+
+def test_c3_band_fused_f32_within_tolerance():
+    """128^3, 20 steps, band rebuilt every 10: the FP32 fused band step against
+    the FP64 drop-in chain on the same initial field, at level-0 points of the
+    last band: L-infinity of phi below 1e-5 and its mean below 1e-7 (h =
+    1/128; the largest differences sit at the kinks of the minimum over
+    spheres, where WENO5's weights turn on rounding: 2.5e-6 measured), and
+    every dt within 1e-5."""
+    from lsmlib_amd.solver import Grid, NarrowBandRun, phi_spheres_min
+    phi0, _ = phi_spheres_min(Grid((128, 128, 128)), 3, 64, (0.1, 0.9), (0.03, 0.1))
+    r64 = NarrowBandRun(n=128, steps=20, rebuild_every=10, phi0=phi0)
+    r32 = NarrowBandRun(n=128, steps=20, rebuild_every=10, phi0=phi0, precision="f32", fused=True)
+    p64, p32 = r64.solver.host(r64.run()), r32.solver.host(r32.run())
+    # level-0 points of both last bands (a point within FP32 rounding of the
+    # band width may fall on either side of it)
+    lv0 = (r64.nb.cpu().numpy() == 1) & (r32.nb.cpu().numpy() == 1)
+    d = np.abs(p64 - p32)[lv0]
+    err, mean = float(d.max()), float(d.mean())
+    print("FP32 fused band vs FP64 drop-in: L-inf %.3e, mean %.3e over %d level-0 points" % (err, mean, d.size))
+    assert err < 1e-5
+    assert mean < 1e-7
+    assert max(abs(a / b - 1) for a, b in zip(r32.dts, r64.dts)) < 1e-5
+
+
+def test_fused_band_stage_reports_a_stencil_outside_the_band():
+    """A band grown to one level only: WENO5 stencils of level-0 points reach
+    cells outside the band (mark 0), where the drop-in chain would read stale
+    differences; the fused stage skips them and reports an index error."""
+    from lsmlib_amd.lib import LsmHipError
+    from lsmlib_amd.solver import NarrowBandRun
+    r = NarrowBandRun(n=48, steps=1, rebuild_every=5, level=1, fused=True)
+    with pytest.raises(LsmHipError, match="error 4"):
+        r.step()
+    r.solver.lib.clear_error()
 
 
 def test_c3_band_full_is_bitwise_the_reference():
