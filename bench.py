@@ -42,6 +42,22 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def _stream_ordered(lib):
+    """Calls of this library go to torch's current stream without waiting for
+    each launch (lsmhip_set_stream(stream, 0)); results needed on the host
+    (CFL steps, band sizes) still synchronise.  Kernels and ordering are the
+    same as in the default synchronous mode."""
+    import torch
+    lib.set_stream(torch.cuda.current_stream().cuda_stream, False)
+
+
+def _device_errors(lib):
+    """Device-side index errors of stream-ordered calls (raised as in the
+    synchronous mode)."""
+    if lib.cdll.lsmhip_check_device_errors() != 0:
+        lib.check()
+
+
 def run_timed(run, steps):
     _sync()
     t0 = time.perf_counter()
@@ -263,6 +279,7 @@ def band_table(n, steps, warmup):
     rows = []
     for prec, fused in (("f64", False), ("f32", False), ("f64", True), ("f32", True)):
         run = NarrowBandRun(n, precision=prec, steps=steps, rebuild_every=1 << 30, phi0=phi0, fused=fused)
+        _stream_ordered(run.solver.lib)
         _sync()
         t0 = time.perf_counter()
         run.rebuild()
@@ -277,6 +294,7 @@ def band_table(n, steps, warmup):
             run.step()
         _sync()
         el = time.perf_counter() - t0
+        _device_errors(run.solver.lib)
         run.solver.lib.check()
         lv0 = int(run.n_hi[0] - run.n_lo[0] + 1)
         lv3 = int(run.n_hi[run.level] - run.n_lo[0] + 1)
@@ -339,11 +357,13 @@ def main():
         del src, dst
     print("[bench] building the %d^3 deformation workload (FP32 fused)" % a.n, file=sys.stderr, flush=True)
     run32 = DeformationRun(a.n, "f32")
+    _stream_ordered(run32.solver.lib)
     for _ in range(a.warmup):
         run32.step()
     # warm-up steps advance time; restart the clock of the field from the same state
     run32.events = []
     el, updates = run_timed(run32, a.steps)
+    _device_errors(run32.solver.lib)
     run32.solver.lib.check()
     value = run32.grid.cells * updates / el
     roof = roofline_from_events(run32, 4)
@@ -362,9 +382,11 @@ def main():
     if not a.no_compare:
         torch.cuda.empty_cache()
         run64 = DeformationRun(a.n, "f64")
+        _stream_ordered(run64.solver.lib)
         for _ in range(a.warmup):
             run64.step()
         el64, up64 = run_timed(run64, a.steps)
+        _device_errors(run64.solver.lib)
         run64.solver.lib.check()
         v64 = run64.grid.cells * up64 / el64
         print("[bench] FP64 fused: %.3f ms/step" % (1e3 * el64 / a.steps), file=sys.stderr, flush=True)
@@ -386,6 +408,7 @@ def main():
         for _ in range(1):
             chain.step()
         elc, upc = run_timed(chain, ksteps)
+        _device_errors(chain.solver.lib)
         chain.solver.lib.check()
         vchain = chain.grid.cells * upc / elc
         print("[bench] FP64 drop-in chain: %.3f ms/step" % (1e3 * elc / ksteps), file=sys.stderr, flush=True)

@@ -149,7 +149,7 @@ __device__ __forceinline__ void pass_through(const LocalStage& a, long long t, i
 
 template <int STAGE>
 __global__ void __launch_bounds__(kT) local_normal_stage_kernel(LocalStage a, int* flag) {
-  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  const long long t = xcd_entry();
   if (t > (long long)a.L.hi - a.L.lo) {
     pass_through<STAGE>(a, t, flag);
     return;
@@ -260,8 +260,7 @@ LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LS
   a.c2 = c2 * c2 * c2;
   int* flag = device_flag();
   if (!flag) return -1;
-  const long long n = (long long)hi - nlo_index0 + 1;
-  const dim3 grid((unsigned)((n + kT - 1) / kT));
+  const dim3 grid(xcd_grid((long long)hi - nlo_index0 + 1));
   switch (stage) {
     case 1: hipLaunchKernelGGL(local_normal_stage_kernel<1>, grid, dim3(kT), 0, stream(), a, flag); break;
     case 2: hipLaunchKernelGGL(local_normal_stage_kernel<2>, grid, dim3(kT), 0, stream(), a, flag); break;
