@@ -271,12 +271,15 @@ def band_table(n, steps, warmup):
     FP32: time of one band construction (level 0 compaction, levels 1..3,
     boundary layers) and of a step (CFL, three stages, ghost fills), band
     sizes, and band cell-updates/s = level-0 points x steps / time; the step
-    through the drop-in local chain and through the fused band stages."""
+    through the drop-in local chain and through the fused band stages; and the
+    FP32 fused field against the FP64 fused one at the level-0 points of both
+    final bands (the same initial field and steps)."""
     import torch
     from lsmlib_amd.solver import Grid, NarrowBandRun, phi_spheres_min
     g = Grid((n, n, n))
     phi0, _ = phi_spheres_min(g, 3, 64, (0.1, 0.9), (0.03, 0.1))
     rows = []
+    finals = {}
     for prec, fused in (("f64", False), ("f32", False), ("f64", True), ("f32", True)):
         run = NarrowBandRun(n, precision=prec, steps=steps, rebuild_every=1 << 30, phi0=phi0, fused=fused)
         _stream_ordered(run.solver.lib)
@@ -302,13 +305,22 @@ def band_table(n, steps, warmup):
                "path": "fused band stages" if fused else "drop-in local chain",
                "band_level0_points": lv0, "band_points_levels_0_3": lv3, "band_build_ms": round(1e3 * t_band, 3),
                "ms_per_step": round(1e3 * el / steps, 3), "band_cell_updates_per_s": round(lv0 * steps / el, 1)}
+        if fused:  # final field and level-0 marks, for the FP32 vs FP64 comparison
+            finals[prec] = (run.solver.host(run.phi), run.nb.cpu().numpy() == 1)
         rows.append(row)
         print("[bench] band %s %s: build %.2f ms, %.3f ms/step, %d level-0 points, %.3e band cell-updates/s" % (
             prec, row["path"], row["band_build_ms"], row["ms_per_step"], lv0, row["band_cell_updates_per_s"]),
             file=sys.stderr, flush=True)
         del run
         torch.cuda.empty_cache()
-    return rows
+    (p64, b64), (p32, b32) = finals["f64"], finals["f32"]
+    both = b64 & b32
+    d = np.abs(p64 - p32)[both]
+    acc = {"steps": warmup + steps, "points": int(both.sum()), "linf_phi_level0": float(d.max()),
+           "mean_abs_phi_level0": float(d.mean())}
+    print("[bench] band FP32 fused vs FP64 fused after %d steps: L-inf %.3e, mean %.3e" % (
+        acc["steps"], acc["linf_phi_level0"], acc["mean_abs_phi_level0"]), file=sys.stderr, flush=True)
+    return rows, acc
 
 
 def main():
@@ -345,9 +357,10 @@ def main():
                           "data": "synthetic", "kernels": rows}))
         return
     if a.band:
-        rows = band_table(a.n, a.steps, a.warmup)
+        rows, acc = band_table(a.n, a.steps, a.warmup)
         print(json.dumps({"metric": "narrow-band local step @%d^3" % a.n, "unit": "band cell-updates/s",
-                          "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows}))
+                          "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows,
+                          "accuracy_fp32_vs_fp64": acc}))
         return
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

@@ -115,10 +115,10 @@ int lsmhip_weno5_local_stage_normal(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi
                                     int nhi_index0, int nhi_copy, const unsigned char *narrow_band,
                                     LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb, unsigned char mark_D1,
                                     LSMLIB_REAL beta, LSMLIB_REAL gamma);
-/* HJ_WENO5_LOCAL -> COMPUTE_STABLE_[CONST_]NORMAL_VEL_DT_LOCAL: dt = cfl / (max
- * over level-0 points of |V| / |grad phi|_max+ * (phi_x/dx + ...) + tiny) with the
- * whole-grid formula (no |grad phi| filter).  FP64: bitwise equal to that chain
- * for constant vn. */
+/* HJ_WENO5_LOCAL -> COMPUTE_STABLE_NORMAL_VEL_DT_LOCAL (vn_field on gb; only
+ * points with |grad phi| > zero_tol count) or, with vn_field NULL,
+ * COMPUTE_STABLE_CONST_NORMAL_VEL_DT_LOCAL in its whole-grid form: dt = cfl /
+ * (max over level-0 points of H/dx + tiny).  FP64: bitwise equal to that chain. */
 int lsmhip_weno5_local_normal_dt(LSMLIB_REAL *dt, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb), LSMLIB_REAL vn,
                                  const LSMLIB_REAL *vn_field, LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
                                  LSMLIB_REAL cfl_number, const int *index_x, const int *index_y,

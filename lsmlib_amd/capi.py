@@ -363,7 +363,10 @@ class _Fn:
                 keep.append(c)
                 cargs.append(ctypes.byref(c))
             else:  # R, IA, IH, M: addresses
-                cargs.append(ctypes.c_void_p(_addr(a)))
+                ptr = _addr(a)
+                if not ptr:  # every array of the table is required; a NULL one would be dereferenced
+                    raise ValueError("array argument %s is NULL" % name)
+                cargs.append(ctypes.c_void_p(ptr))
         self._f(*cargs)
         vals = tuple(o.value for o in outs)
         if not vals:

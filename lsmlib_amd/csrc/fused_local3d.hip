@@ -183,8 +183,8 @@ __global__ void __launch_bounds__(kT) local_normal_stage_kernel(LocalStage a, in
   a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, r);
 }
 
-// CFL of normal motion at one level-0 point (lsm3dComputeStableNormalVelDtLOCAL's
-// H over dx with the constant-speed form when vnf is null, no |grad phi| filter).
+// CFL of normal motion at one level-0 point: lsm3dComputeStableNormalVelDtLOCAL
+// with a speed field, the whole-grid constant-speed formula otherwise.
 struct LocalDtOp {
   LocalStage a;
   real_t mdx2;
@@ -197,8 +197,11 @@ struct LocalDtOp {
     if (!derivatives_local(a, i, j, k, p, g, flag)) return false;
     const real_t av = a.vnf ? fabs(a.vnf[p]) : fabs(a.vn);
     real_t n;
-    r = (double)stencil::normal_h_over_dx(av, g[0], g[1], g[2], g[3], g[4], g[5], mdx2, a.inv[0], a.inv[1], a.inv[2],
-                                          &n);
+    const real_t h = stencil::normal_h_over_dx(av, g[0], g[1], g[2], g[3], g[4], g[5], mdx2, a.inv[0], a.inv[1],
+                                               a.inv[2], &n);
+    // the variable-speed routine admits only points with |grad phi| > zero_tol
+    // (lsm_math_utils3d_local.f.in:405-416); the constant-speed one all points
+    r = (a.vnf && !(n > kZeroTol)) ? -1.0 : (double)h;
     return true;
   }
 };
@@ -245,6 +248,10 @@ LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LS
     set_error(kErrArg, "%s: stage %d needs phi_n", who, stage);
     return -1;
   }
+  if (!phi_out || !phi_stage || !index_x || !index_y || !index_z || !narrow_band) {
+    set_error(kErrArg, "%s: a required array is NULL", who);
+    return -1;
+  }
   if (!check_size(gb, who) || !check_size(nbb, who)) return -1;
   const int hi = nhi_copy > nhi_index0 ? nhi_copy : nhi_index0;
   if (hi < nlo_index0) return 0;
@@ -282,6 +289,10 @@ LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt(LSMLIB_REAL* dt, const LSMLIB_REA
   const char* who = "lsmhip_weno5_local_normal_dt";
   const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
   const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
+  if (!dt || !phi || !index_x || !index_y || !index_z || !narrow_band) {
+    set_error(kErrArg, "%s: a required array is NULL", who);
+    return -1;
+  }
   if (!check_size(gb, who) || !check_size(nbb, who)) return -1;
   int* flag = device_flag();
   if (!flag) return -1;
