@@ -264,6 +264,123 @@ def kernel_table(n, reps, warmup):
     return rows
 
 
+# 2D entry points timed one by one on an n^2 grid: (C-ABI name, real arrays
+# read per cell, real arrays written per cell).
+KERNELS_2D = [
+    ("lsm2dhjweno5", 1, 4),
+    ("lsm2dupwindhjweno5", 3, 2),
+    ("lsm2dcentralgradorder2", 1, 2),
+    ("lsm2dgradientmagnitude", 2, 1),
+    ("lsm2daddnormalveltermtolserhs", 6, 1),
+    ("lsm2dcomputereinitializationeqnrhs", 5, 1),
+    ("lsm2dcomputemeancurvatureorder2", 4, 1),
+    ("lsm2dtvdrk3stage2", 3, 1),
+]
+
+
+def kernel_table_2d(n, reps, warmup):
+    """Average launch time (HIP events, `reps` back-to-back launches) and HBM
+    roofline fraction of each KERNELS_2D entry point on an n^2 grid with 3
+    ghost cells (circle of radius 0.3: phi its signed distance, exact
+    gradients), FP64 and FP32; then the 2D band of half-width 6h around that
+    circle: its construction (levels 0..3) and HJ-WENO5 on the band, whose
+    algorithmic bytes per level-0 point are 8 B of index, 1 B of mark, phi and
+    four outputs."""
+    import torch
+    import lsmlib_amd
+    rows = []
+    g = 3
+    gb = (-g, n - 1 + g) * 2
+    fb = (0, n - 1) * 2
+    h = 1.0 / n
+    cells = n * n
+    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        lib = lsmlib_amd.load(prec)
+        stream = torch.cuda.current_stream()
+        lib.set_stream(stream.cuda_stream, False)
+        ax = (torch.arange(-g, n + g, dtype=dt, device="cuda") + 0.5) * h - 0.5
+        y, x = torch.meshgrid(ax, ax, indexing="ij")
+        r = torch.sqrt(x * x + y * y)
+        P = {"phi": r - 0.3, "px": x / r, "py": y / r}
+        P["gmag"] = torch.ones_like(r)
+        P["vn"] = 0.5 + 0.2 * torch.sin(8 * x)
+        del y, x, r
+        o = [torch.zeros_like(P["phi"]) for _ in range(4)]
+        D1 = torch.zeros_like(P["phi"])
+        size = 8 if prec == "f64" else 4
+        calls = {
+            "lsm2dhjweno5": (o[0], o[1], gb, o[2], o[3], gb, P["phi"], gb, D1, gb, fb, h, h),
+            "lsm2dupwindhjweno5": (o[0], o[1], gb, P["phi"], gb, P["px"], P["py"], gb, D1, gb, fb, h, h),
+            "lsm2dcentralgradorder2": (o[0], o[1], gb, P["phi"], gb, fb, h, h),
+            "lsm2dgradientmagnitude": (P["px"], P["py"], o[0], gb, fb),
+            "lsm2daddnormalveltermtolserhs": (o[0], gb, P["px"], P["py"], gb, P["gmag"], P["vn"], gb, P["phi"], gb,
+                                              fb),
+            "lsm2dcomputereinitializationeqnrhs": (o[0], gb, P["phi"], gb, P["phi"], gb, P["px"], P["py"], gb,
+                                                   P["gmag"], P["vn"], gb, fb, h, h, 0),
+            "lsm2dcomputemeancurvatureorder2": (o[0], gb, P["phi"], gb, P["px"], P["py"], P["gmag"], gb, fb, h, h),
+            "lsm2dtvdrk3stage2": (o[0], gb, P["px"], gb, P["phi"], gb, P["py"], gb, fb, 0.25 * h),
+        }
+        for name, rd, wr in KERNELS_2D:
+            fn, args = getattr(lib.api, name), calls[name]
+            for _ in range(warmup):
+                fn(*args)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(*args)
+            e1.record(stream)
+            e1.synchronize()
+            _device_errors(lib)
+            lib.check()
+            ms = e0.elapsed_time(e1) / reps
+            alg = (rd + wr) * cells * size
+            gbs = alg / (ms * 1e-3) / 1e9
+            rows.append({"kernel": name, "dtype": prec, "avg_launch_ms": round(ms, 4), "alg_bytes": alg,
+                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            print("[bench] %-38s %s %8.4f ms %7.1f GB/s %5.1f%% of HBM peak" % (
+                name, prec, ms, gbs, 100 * gbs / HBM_PEAK_GBS), file=sys.stderr, flush=True)
+        # the band: construction, then HJ-WENO5 over it
+        phi = P["phi"]
+        npts = phi.numel()
+        nb = torch.zeros(phi.shape, dtype=torch.uint8, device="cuda")
+        ix, iy, outer = (torch.zeros(npts, dtype=torch.int32, device="cuda") for _ in range(3))
+        n_lo, n_hi = np.zeros(4, np.int32), np.zeros(4, np.int32)
+        _sync()
+        t0 = time.perf_counter()
+        lib.api.lsm2ddeterminenarrowband(phi, gb, nb, gb, ix, iy, 0, npts - 1, n_lo, n_hi, outer, 0, npts - 1,
+                                         None, None, None, None, 6 * h, 3 * h, 3)
+        for box, mark in (((-1, n, -1, n), 125), ((-2, n + 1, -2, n + 1), 126), (gb, 127)):
+            lib.api.lsm2dmarknarrowbandboundarylayer(nb, gb, box, mark)
+        _sync()
+        t_band = time.perf_counter() - t0
+        lv = [int(v) for pair in zip(n_lo, n_hi) for v in pair]
+        args = (o[0], o[1], gb, o[2], o[3], gb, phi, gb, D1, gb, h, h, ix, iy, *lv, nb, gb, 124, 126)
+        for _ in range(warmup):
+            lib.api.lsm2dhjweno5local(*args)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            lib.api.lsm2dhjweno5local(*args)
+        e1.record(stream)
+        e1.synchronize()
+        _device_errors(lib)
+        lib.check()
+        ms = e0.elapsed_time(e1) / reps
+        lv0 = lv[1] - lv[0] + 1
+        alg = lv0 * (8 + 1 + 5 * size)
+        gbs = alg / (ms * 1e-3) / 1e9
+        rows.append({"kernel": "lsm2dhjweno5local (band half-width 6h, levels 0..3)", "dtype": prec,
+                     "band_level0_points": lv0, "band_build_ms": round(1e3 * t_band, 3),
+                     "avg_launch_ms": round(ms, 4), "alg_bytes": alg, "achieved": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        print("[bench] 2D band %s: build %.3f ms, %d level-0 points, WENO5 local %.4f ms, %.1f GB/s" % (
+            prec, 1e3 * t_band, lv0, ms, gbs), file=sys.stderr, flush=True)
+        lib.set_stream(0, True)
+        del P, o, D1, phi, nb, ix, iy, outer
+        torch.cuda.empty_cache()
+    return rows
+
+
 def band_table(n, steps, warmup):
     """The narrow-band workload (64 spheres at n^3, band 6dx / 3dx, levels
     0..3, local WENO5 + constant normal speed + cut-off + local TVD-RK3, as
@@ -336,6 +453,8 @@ def main():
     ap.add_argument("--kernels", action="store_true",
                     help="time the normal / curvature / orthogonalization / field-extension entry points one by "
                          "one at n^3 (--steps launches each) instead of the level-set step")
+    ap.add_argument("--kernels2d", action="store_true",
+                    help="time 2D entry points (whole grid, and HJ-WENO5 on a 2D band) one by one on an n^2 grid")
     ap.add_argument("--band", action="store_true",
                     help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
                          "instead of the whole-grid step")
@@ -353,6 +472,12 @@ def main():
     if a.kernels:
         rows = kernel_table(a.n, a.steps, a.warmup)
         print(json.dumps({"metric": "drop-in entry point HBM roofline @%d^3" % a.n, "unit": "GB/s",
+                          "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                          "data": "synthetic", "kernels": rows}))
+        return
+    if a.kernels2d:
+        rows = kernel_table_2d(a.n, a.steps, a.warmup)
+        print(json.dumps({"metric": "2D drop-in entry point HBM roofline @%d^2" % a.n, "unit": "GB/s",
                           "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                           "data": "synthetic", "kernels": rows}))
         return

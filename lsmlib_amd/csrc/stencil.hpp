@@ -41,6 +41,18 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5, T tiny = T(1.e-
   return a1 * p1 + a2 * p2 + a3 * p3;
 }
 
+// In FP32 the rearranged reconstruction of the fused stages (fastf::weno5
+// below: same weights and candidates, fewer instructions, different rounding)
+// serves every WENO5 routine; the additive tiny term only matters when every
+// |v| < 1e-15, where all candidates vanish.
+namespace fastf {
+__device__ float weno5(float v1, float v2, float v3, float v4, float v5);
+}
+__device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5, float tiny = 1.e-36f) {
+  (void)tiny;
+  return fastf::weno5(v1, v2, v3, v4, v5);
+}
+
 // Undivided differences from seven samples f[s+3] = phi(offset s) along an axis:
 // D1(s) = f(s) - f(s-1) (face s-1/2 stored at s), D2(s) = -(D1(s) - D1(s+1)),
 // D3(s) = D2(s) - D2(s-1)  (lsmComputeDn, lsm_spatial_derivatives3d.f.in:49-175).
