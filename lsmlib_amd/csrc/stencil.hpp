@@ -254,9 +254,9 @@ __device__ __forceinline__ T normal_h_over_dx(T absv, T xp, T yp, T zp, T xm, T 
 }
 
 // ---------------------------------------------------------------------------
-// FP32 fused-path forms: the same HJ-WENO5 scheme rearranged for fewer VALU
-// instructions (FP32 results are judged by their error against FP64, not by
-// bitwise equality).
+// FP32 forms: the same HJ-WENO5 scheme rearranged for fewer VALU instructions
+// (FP32 results are held to an error bound against FP64, not to bitwise
+// equality).
 //  * The reconstruction is evaluated on undivided differences and scaled by
 //    1/dx once: the weights are invariant under a common scaling of v.
 //  * With m = max(v_k^2) the weights alpha_k = d_k / (S_k + 1e-6 m)^2 equal
