@@ -174,3 +174,19 @@ def test_circle_closed_forms_4096(hip):
     assert e_k < 1e-6
     assert abs(area / (np.pi * R * R) - 1) < 1e-6
     assert abs(perim / (2 * np.pi * R) - 1) < 2e-5
+
+
+@pytest.mark.parametrize("scheme", ["eno1", "eno2", "eno3", "weno5"])
+def test_neumann_zero_normal_derivative_2d_hip(hip, scheme):
+    """tests/boundary_conditions/test_neumann_bc_2d.cc on the device: bitwise
+    the oracle's, and the face-normal one-sided derivative below 1e-6 on every
+    boundary line."""
+    from test_oracle_pinning import neumann_fill_and_derivatives2d
+    ref = neumann_fill_and_derivatives2d(H.oracle(), scheme)
+    got = neumann_fill_and_derivatives2d(hip.api, scheme, device="cuda")
+    hip.check()
+    H.assert_same(ref, got, scheme, skip=("D1", "D2", "D3"))
+    a = got[0]
+    I = (slice(3, -3),) * 2
+    assert np.abs(a["pxm"][I][:, 0]).max() < 1e-6 and np.abs(a["pxp"][I][:, -1]).max() < 1e-6
+    assert np.abs(a["pym"][I][0, :]).max() < 1e-6 and np.abs(a["pyp"][I][-1, :]).max() < 1e-6

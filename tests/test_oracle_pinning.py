@@ -89,6 +89,45 @@ def test_neumann_zero_normal_derivative(scheme):
     assert np.abs(I(zm)[0, :, :]).max() < 1e-6 and np.abs(I(zp)[-1, :, :]).max() < 1e-6
 
 
+def neumann_case2d():
+    """The fixture of the reference's tests/boundary_conditions/test_neumann_bc_2d.cc:
+    25 x 20 cells of [0,1]^2 (cell centres), 3 ghost cells,
+    phi = (x-0.25)^2 + 4 (y-0.35)^2 - 0.25 in the interior (ghosts zero here)."""
+    n, g = (25, 20), 3
+    c = H.Case((-g, n[0] - 1 + g, -g, n[1] - 1 + g), (0, n[0] - 1, 0, n[1] - 1))
+    c.boxes["gb"] = c.gb
+    dx, dy = 1.0 / n[0], 1.0 / n[1]
+    c.scalars.update(dx=dx, dy=dy)
+    phi = np.zeros(H.shape_of(c.gb))
+    j, i = np.meshgrid(np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    x, y = (i + 0.5) * dx, (j + 0.5) * dy
+    phi[g:g + n[1], g:g + n[0]] = (x - 0.25) ** 2 + 4 * (y - 0.35) ** 2 - 0.25
+    c.arrays["phi"] = phi
+    for nm in ("pxp", "pyp", "pxm", "pym", "D1", "D2", "D3"):
+        c.arrays[nm] = np.zeros(H.shape_of(c.gb))
+    return c
+
+
+def neumann_fill_and_derivatives2d(impl, scheme, device=None):
+    c = neumann_case2d()
+    for idx in range(4):
+        c.scalars["idx"] = idx
+        arrs, _ = H.run(impl, "lsm2dhomogeneousneumann%s_" % scheme, c, device=device)
+        c.arrays["phi"] = arrs["phi"]
+    return H.run(impl, "lsm2dhj%s_" % scheme, c, device=device)
+
+
+@pytest.mark.parametrize("scheme", ["eno1", "eno2", "eno3", "weno5"])
+def test_neumann_zero_normal_derivative_2d(scheme):
+    """tests/boundary_conditions/test_neumann_bc_2d.cc (:109-600): after a
+    homogeneous Neumann fill of the four faces the face-normal one-sided
+    derivative vanishes on every boundary line (tolerance 1e-6)."""
+    a, _ = neumann_fill_and_derivatives2d(O(), scheme)
+    I = (slice(3, -3),) * 2
+    assert np.abs(a["pxm"][I][:, 0]).max() < 1e-6 and np.abs(a["pxp"][I][:, -1]).max() < 1e-6
+    assert np.abs(a["pym"][I][0, :]).max() < 1e-6 and np.abs(a["pyp"][I][-1, :]).max() < 1e-6
+
+
 # ---------------------------------------------------------------------------
 # Closed forms
 # ---------------------------------------------------------------------------
