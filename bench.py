@@ -17,6 +17,10 @@ FP64 Kahan-compensated).  Inputs are resident in HBM when timing starts.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-compare]
     python bench.py --kernels [--n 512] [--steps K]   # normals, curvatures, orthogonalization,
                                                       # field extension: HBM roofline per entry point
+    python bench.py --core [--n 512] [--steps K]      # whole-grid derivatives, RHS terms, RK stages,
+                                                      # CFL and integrals: HBM roofline per entry point
+    python bench.py --kernels2d [--n 8192]            # 2D entry points and a 2D band
+    python bench.py --band [--n 512]                  # narrow-band step: drop-in chain and fused stages
 """
 from __future__ import annotations
 
@@ -381,6 +385,137 @@ def kernel_table_2d(n, reps, warmup):
     return rows
 
 
+# (entry point, arrays read, arrays written) per fill-box cell; the reductions
+# write one host scalar.  The HJ scratch arrays (D1, D2) are neither read nor
+# written by these kernels and are not counted; the surface integral reads the
+# gradient only where |phi| < eps (under 1 % of the cells here), so only phi
+# is counted for it.
+CORE_KERNELS = [
+    ("lsm3dhjweno5", 1, 6),
+    ("lsm3dhjeno2", 1, 6),
+    ("lsm3dupwindhjweno5", 4, 3),
+    ("lsm3dcentralgradorder2", 1, 3),
+    ("lsm3dlaplacianorder2", 1, 1),
+    ("lsm3dgradientmagnitude", 3, 1),
+    ("lsm3dzerooutlevelseteqnrhs", 0, 1),
+    ("lsm3daddadvectiontermtolserhs", 7, 1),
+    ("lsm3daddnormalveltermtolserhs", 8, 1),
+    ("lsm3daddconstnormalveltermtolserhs", 7, 1),
+    ("lsm3daddexternalandnormalveltermtolserhs", 11, 1),
+    ("lsm3dcomputereinitializationeqnrhs", 7, 1),
+    ("lsm3dcomputemeancurvatureorder2", 5, 1),
+    ("lsm3daddconstcurvtermtolserhs", 10, 1),
+    ("lsm3daddconstprecomputedcurvtermtolserhs", 3, 1),
+    ("lsm3dtvdrk3stage1", 2, 1),
+    ("lsm3dtvdrk3stage2", 3, 1),
+    ("lsm3dtvdrk3stage3", 3, 1),
+    ("lsm3dcomputestableadvectiondt", 3, 0),
+    ("lsm3dcomputestablenormalveldt", 7, 0),
+    ("lsm3dmaxnormdiff", 2, 0),
+    ("lsm3dvolumeregionphilessthanzero", 1, 0),
+    ("lsm3dsurfaceareazerolevelset", 1, 0),
+]
+
+
+def core_table(n, reps, warmup):
+    """Average launch time (HIP events on the launch stream, `reps` calls back
+    to back) and HBM roofline fraction of each CORE_KERNELS entry point -- the
+    whole-grid derivatives, RHS terms, RK stages, CFL and integral reductions
+    of the time-stepping path -- at n^3 with 3 ghost cells, FP64 and FP32, on
+    a sphere of radius 0.3: phi its signed distance, the plus and minus
+    gradients exact and in distinct arrays, a rotating velocity field and a
+    varying normal speed.  Every array is distinct, so no read is served by
+    another array's cache lines.  The reductions return a host scalar and
+    their time includes that read-back."""
+    import torch
+    import lsmlib_amd
+    rows = []
+    g = 3
+    gb = (-g, n - 1 + g) * 3
+    fb = (0, n - 1) * 3
+    h = 1.0 / n
+    cells = n ** 3
+    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        lib = lsmlib_amd.load(prec)
+        stream = torch.cuda.current_stream()
+        lib.set_stream(stream.cuda_stream, False)
+        ax = (torch.arange(-g, n + g, dtype=dt, device="cuda") + 0.5) * h - 0.5
+        z, y, x = torch.meshgrid(ax, ax, ax, indexing="ij")
+        r = torch.sqrt(x * x + y * y + z * z)
+        A = {"phi": r - 0.3, "px": x / r, "py": y / r, "pz": z / r}
+        A["vx"], A["vy"], A["vz"] = -y + 0.1, x - 0.2, 0.3 * torch.sin(6 * z)
+        A["vn"] = 0.5 + 0.2 * torch.sin(8 * x)
+        del z, y, x
+        for k in ("px", "py", "pz"):
+            A["m" + k] = A[k].clone()
+        A["gmag"] = torch.ones_like(r)
+        d = (A["px"], A["py"], A["pz"])
+        for key, (p, q) in {"xx": (0, 0), "xy": (0, 1), "xz": (0, 2), "yy": (1, 1), "yz": (1, 2),
+                            "zz": (2, 2)}.items():
+            A["p" + key] = ((1.0 if p == q else 0.0) - d[p] * d[q]) / r
+        A["kappa"] = 2.0 / r
+        A["rhs0"] = 0.1 * A["vn"]
+        del r, d
+        o = [torch.zeros_like(A["phi"]) for _ in range(6)]
+        D1 = torch.zeros_like(A["phi"])
+        P = A
+        plus = (P["px"], P["py"], P["pz"], gb)
+        minus = (P["mpx"], P["mpy"], P["mpz"], gb)
+        vel = (P["vx"], P["vy"], P["vz"], gb)
+        calls = {
+            "lsm3dhjweno5": (*o[:3], gb, *o[3:], gb, P["phi"], gb, D1, gb, fb, h, h, h),
+            "lsm3dhjeno2": (*o[:3], gb, *o[3:], gb, P["phi"], gb, D1, gb, D1, gb, fb, h, h, h),
+            "lsm3dupwindhjweno5": (*o[:3], gb, P["phi"], gb, *vel, D1, gb, fb, h, h, h),
+            "lsm3dcentralgradorder2": (*o[:3], gb, P["phi"], gb, fb, h, h, h),
+            "lsm3dlaplacianorder2": (o[0], gb, P["phi"], gb, fb, h, h, h),
+            "lsm3dgradientmagnitude": (P["px"], P["py"], P["pz"], o[0], gb, fb),
+            "lsm3dzerooutlevelseteqnrhs": (o[0], gb),
+            "lsm3daddadvectiontermtolserhs": (o[0], gb, *plus, *vel, fb),
+            "lsm3daddnormalveltermtolserhs": (o[0], gb, *plus, *minus, P["vn"], gb, fb),
+            "lsm3daddconstnormalveltermtolserhs": (o[0], gb, *plus, *minus, 0.75, fb),
+            "lsm3daddexternalandnormalveltermtolserhs": (o[0], gb, *plus, *minus, P["vn"], *vel, fb),
+            "lsm3dcomputereinitializationeqnrhs": (o[0], gb, P["phi"], gb, P["phi"], gb, *plus, *minus, fb, h, h, h,
+                                                   0),
+            "lsm3dcomputemeancurvatureorder2": (o[0], gb, P["phi"], gb, P["px"], P["py"], P["pz"], P["gmag"], gb, fb,
+                                                h, h, h),
+            "lsm3daddconstcurvtermtolserhs": (o[0], gb, *plus, P["pxx"], P["pxy"], P["pxz"], P["pyy"], P["pyz"],
+                                              P["pzz"], gb, -0.02, fb),
+            "lsm3daddconstprecomputedcurvtermtolserhs": (o[0], gb, P["kappa"], gb, P["gmag"], gb, -0.02, fb),
+            "lsm3dtvdrk3stage1": (o[0], gb, P["phi"], gb, P["rhs0"], gb, fb, 0.25 * h),
+            "lsm3dtvdrk3stage2": (o[0], gb, P["px"], gb, P["phi"], gb, P["rhs0"], gb, fb, 0.25 * h),
+            "lsm3dtvdrk3stage3": (o[0], gb, P["py"], gb, P["phi"], gb, P["rhs0"], gb, fb, 0.25 * h),
+            "lsm3dcomputestableadvectiondt": (None, *vel, fb, h, h, h, 0.5),
+            "lsm3dcomputestablenormalveldt": (None, P["vn"], gb, *plus, *minus, fb, h, h, h, 0.5),
+            "lsm3dmaxnormdiff": (None, P["phi"], gb, P["px"], gb, fb),
+            "lsm3dvolumeregionphilessthanzero": (None, P["phi"], gb, fb, h, h, h, 1.5 * h),
+            "lsm3dsurfaceareazerolevelset": (None, P["phi"], gb, *plus, fb, h, h, h, 1.5 * h),
+        }
+        size = 8 if prec == "f64" else 4
+        for name, rd, wr in CORE_KERNELS:
+            fn, args = getattr(lib.api, name), calls[name]
+            for _ in range(warmup):
+                fn(*args)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                fn(*args)
+            e1.record(stream)
+            e1.synchronize()
+            _device_errors(lib)
+            lib.check()
+            ms = e0.elapsed_time(e1) / reps
+            alg = (rd + wr) * cells * size
+            gbs = alg / (ms * 1e-3) / 1e9
+            rows.append({"kernel": name, "dtype": prec, "avg_launch_ms": round(ms, 4), "alg_bytes": alg,
+                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+            print("[bench] %-42s %s %8.4f ms %7.1f GB/s %5.1f%% of HBM peak" % (
+                name, prec, ms, gbs, 100 * gbs / HBM_PEAK_GBS), file=sys.stderr, flush=True)
+        lib.set_stream(0, True)
+        del A, P, o, D1, plus, minus, vel, calls
+        torch.cuda.empty_cache()
+    return rows
+
+
 def band_table(n, steps, warmup):
     """The narrow-band workload (64 spheres at n^3, band 6dx / 3dx, levels
     0..3, local WENO5 + constant normal speed + cut-off + local TVD-RK3, as
@@ -455,6 +590,9 @@ def main():
                          "one at n^3 (--steps launches each) instead of the level-set step")
     ap.add_argument("--kernels2d", action="store_true",
                     help="time 2D entry points (whole grid, and HJ-WENO5 on a 2D band) one by one on an n^2 grid")
+    ap.add_argument("--core", action="store_true",
+                    help="time the whole-grid derivative, RHS, RK, CFL and integral entry points one by one at n^3 "
+                         "(--steps calls each) instead of the level-set step")
     ap.add_argument("--band", action="store_true",
                     help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
                          "instead of the whole-grid step")
@@ -469,6 +607,12 @@ def main():
     import torch
     from lsmlib_amd.solver import DeformationRun
 
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        _sync()
+        del src, dst
     if a.kernels:
         rows = kernel_table(a.n, a.steps, a.warmup)
         print(json.dumps({"metric": "drop-in entry point HBM roofline @%d^3" % a.n, "unit": "GB/s",
@@ -481,18 +625,18 @@ def main():
                           "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                           "data": "synthetic", "kernels": rows}))
         return
+    if a.core:
+        rows = core_table(a.n, a.steps, a.warmup)
+        print(json.dumps({"metric": "whole-grid time-stepping entry point HBM roofline @%d^3" % a.n,
+                          "unit": "GB/s", "peak": HBM_PEAK_GBS, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                          "data": "synthetic", "kernels": rows}))
+        return
     if a.band:
         rows, acc = band_table(a.n, a.steps, a.warmup)
         print(json.dumps({"metric": "narrow-band local step @%d^3" % a.n, "unit": "band cell-updates/s",
                           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows,
                           "accuracy_fp32_vs_fp64": acc}))
         return
-    if a.calibrate_copy:
-        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        _sync()
-        del src, dst
     print("[bench] building the %d^3 deformation workload (FP32 fused)" % a.n, file=sys.stderr, flush=True)
     run32 = DeformationRun(a.n, "f32")
     _stream_ordered(run32.solver.lib)

@@ -18,11 +18,6 @@ Six six(const real_t* xp, const real_t* yp, const real_t* zp, const Box& gp, con
   return Six{view(xp, gp), view(yp, gp), view(zp, gp), view(xm, gm), view(ym, gm), view(zm, gm)};
 }
 
-__global__ void __launch_bounds__(256) fill_kernel(Box b, View<real_t> a, real_t v) {
-  LSMHIP_CELL(b);
-  a(i, j, k) = v;
-}
-
 __global__ void __launch_bounds__(256) advection_kernel(Box fb, View<real_t> rhs, View<const real_t> px,
                                                         View<const real_t> py, View<const real_t> pz,
                                                         View<const real_t> vx, View<const real_t> vy,
@@ -156,8 +151,8 @@ LSMHIP_EXPORT void lsm3dzerooutlevelseteqnrhs_(real_t* rhs, LSMHIP_BOXP(rg)) {
   const char* who = "lsm3dzerooutlevelseteqnrhs_";
   Box rg = LSMHIP_BOX(rg);
   if (rg.empty() || !check_size(rg, who)) return;
-  hipLaunchKernelGGL(fill_kernel, grid_for(rg), block2d(), 0, stream(), rg, view(rhs, rg), real_t(0));
-  finish(who);
+  // the whole ghost box, as the reference zeroes it: rg.volume() contiguous scalars from rhs
+  if (fill_contiguous(rhs, rg.volume(), real_t(0), who)) finish(who);
 }
 
 LSMHIP_EXPORT void lsm3daddadvectiontermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), const real_t* px, const real_t* py,

@@ -95,6 +95,9 @@ bool synchronous();
 // default, matching the reference's completion semantics), waits for the stream.
 bool finish(const char* who);  // launch status, and completion in synchronous mode
 bool check_hip(hipError_t e, const char* who);
+// Sets n contiguous scalars from a to v on the library's stream (a kernel, so a
+// caller may capture it in a graph; launch status only).
+bool fill_contiguous(real_t* a, long long n, real_t v, const char* who);
 // Device scratch owned by the library, grown on demand (never shrinks).
 void* workspace(size_t bytes);
 // Per-device int flag that index-list kernels raise when an index or a stencil

@@ -50,18 +50,43 @@ __device__ __forceinline__ Pair wave_sum(Pair v) {
   return v;
 }
 
+// Cells are visited one row (fixed j, k) per wave at a time: the lanes stride
+// along i, so loads are coalesced and the row index is split into (j, k) once
+// per row rather than once per cell; each lane issues kUnroll loads before it
+// accumulates, into kUnroll independent accumulators (no serial dependence
+// between them), which the caller combines in a fixed order.  Out-of-row
+// slots contribute the identity (0 for the maxima, whose terms are >= 0; 0 for
+// the sums, which leaves a Neumaier pair as is).
+constexpr int kUnroll = 4;
+
+template <class Op, class Acc, class F>
+__device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[kUnroll], F combine) {
+  const int nx = b.nx(), ny = b.ny();
+  const long long rows = (long long)ny * b.nz();
+  const int lane = threadIdx.x & 63;
+  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  for (long long r = blockIdx.x * (long long)(kThreads / 64) + (threadIdx.x >> 6); r < rows; r += nwaves) {
+    const int j = b.jlo + (int)(r % ny), k = b.klo + (int)(r / ny);
+    for (int i0 = lane; i0 < nx; i0 += 64 * kUnroll) {
+      double v[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        const int i = i0 + 64 * u;
+        v[u] = i < nx ? op(b.ilo + i, j, k) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) acc[u] = combine(acc[u], v[u]);
+    }
+  }
+}
+
 template <class Op>
 __global__ void __launch_bounds__(kThreads) max_kernel(Box b, Op op, unsigned long long* acc) {
-  const long long n = b.volume();
-  const int nx = b.nx(), ny = b.ny();
-  double m = 0.0;
-  for (long long t = blockIdx.x * (long long)kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
-    const int i = b.ilo + (int)(t % nx);
-    const int j = b.jlo + (int)((t / nx) % ny);
-    const int k = b.klo + (int)(t / ((long long)nx * ny));
-    const double v = op(i, j, k);
-    m = v > m ? v : m;
-  }
+  double ms[kUnroll] = {};
+  for_rows(b, op, ms, [](double a, double v) { return v > a ? v : a; });
+  double m = ms[0];
+#pragma unroll
+  for (int u = 1; u < kUnroll; u++) m = ms[u] > m ? ms[u] : m;
   m = wave_max(m);
   __shared__ double part[kThreads / 64];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
@@ -74,15 +99,11 @@ __global__ void __launch_bounds__(kThreads) max_kernel(Box b, Op op, unsigned lo
 
 template <class Op>
 __global__ void __launch_bounds__(kThreads) sum_kernel(Box b, Op op, double* partials) {
-  const long long n = b.volume();
-  const int nx = b.nx(), ny = b.ny();
-  Pair p{0.0, 0.0};
-  for (long long t = blockIdx.x * (long long)kThreads + threadIdx.x; t < n; t += (long long)gridDim.x * kThreads) {
-    const int i = b.ilo + (int)(t % nx);
-    const int j = b.jlo + (int)((t / nx) % ny);
-    const int k = b.klo + (int)(t / ((long long)nx * ny));
-    p = add(p, op(i, j, k));
-  }
+  Pair ps_[kUnroll] = {};
+  for_rows(b, op, ps_, [](Pair a, double v) { return add(a, v); });
+  Pair p = ps_[0];
+#pragma unroll
+  for (int u = 1; u < kUnroll; u++) p = add(p, ps_[u]);
   p = wave_sum(p);
   __shared__ double ps[kThreads / 64], pc[kThreads / 64];
   if ((threadIdx.x & 63) == 0) {
@@ -116,8 +137,9 @@ static __global__ void __launch_bounds__(kThreads) sum_finish_kernel(const doubl
   }
 }
 
+// One wave per row until the block cap.
 inline int blocks_for(const Box& b) {
-  long long n = (b.volume() + kThreads - 1) / kThreads;
+  long long n = ((long long)b.ny() * b.nz() + kThreads / 64 - 1) / (kThreads / 64);
   return (int)(n < kMaxBlocks ? (n < 1 ? 1 : n) : kMaxBlocks);
 }
 

@@ -49,17 +49,20 @@ struct VolumeTerm {
   real_t eps, dV;
   int sign;
   __device__ double operator()(int i, int j, int k) const {
-    const real_t p = phi(i, j, k), q = p / eps;
+    // q = phi / eps only inside the smoothing band, where it is used
+    const real_t p = phi(i, j, k);
     const real_t f = F.p ? F(i, j, k) : real_t(1);
     if (sign < 0) {
       if (p < -eps) return (double)(F.p ? f * dV : dV);
       if (p < eps) {
+        const real_t q = p / eps;
         const real_t h = real_t(0.5) * (real_t(1) - q - real_t(kOneOverPi) * sin(real_t(kPi) * q));
         return (double)(F.p ? h * f * dV : h * dV);
       }
     } else {
       if (p > eps) return (double)(F.p ? f * dV : dV);
       if (p > -eps) {
+        const real_t q = p / eps;
         const real_t h = real_t(0.5) * (real_t(1) + q + real_t(kOneOverPi) * sin(real_t(kPi) * q));
         return (double)(F.p ? h * f * dV : h * dV);
       }

@@ -27,7 +27,35 @@ int current_device() {
   if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) return 0;
   return d;
 }
+
+// n contiguous scalars from a (naturally aligned) set to v: 16-byte stores over
+// the 16-byte-aligned body, one thread per vector; scalar stores for the at
+// most W - 1 elements before the body and after it.
+constexpr int kFillThreads = 256;
+constexpr int kFillW = 16 / (int)sizeof(real_t);
+typedef real_t fill_vec_t __attribute__((ext_vector_type(kFillW)));
+
+__global__ void __launch_bounds__(kFillThreads) fill_contiguous_kernel(real_t* a, long long head, long long nvec,
+                                                                      long long n, real_t v) {
+  const long long t = blockIdx.x * (long long)kFillThreads + threadIdx.x;
+  if (t < nvec) reinterpret_cast<fill_vec_t*>(a + head)[t] = fill_vec_t(v);
+  const long long tail0 = head + nvec * kFillW;
+  if (t < head) a[t] = v;
+  if (t < n - tail0) a[tail0 + t] = v;
+}
 }  // namespace
+
+bool fill_contiguous(real_t* a, long long n, real_t v, const char* who) {
+  if (n <= 0) return true;
+  const long long mis = (long long)(((uintptr_t)a) & 15) / (long long)sizeof(real_t);
+  long long head = mis ? kFillW - mis : 0;
+  if (head > n) head = n;
+  const long long nvec = (n - head) / kFillW;
+  const long long blocks = (nvec + kFillThreads - 1) / kFillThreads;
+  hipLaunchKernelGGL(fill_contiguous_kernel, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(kFillThreads), 0,
+                     g_stream, a, head, nvec, n, v);
+  return check_hip(hipGetLastError(), who);
+}
 
 void set_error(int code, const char* fmt, ...) {
   g_err = code;

@@ -126,11 +126,6 @@ __global__ void __launch_bounds__(256) grad_mag_2d_kernel(Box fb, CV x, CV y, MV
 // ---------------------------------------------------------------------------
 // Level-set RHS terms (lsm_level_set_evolution2d.f.in)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) fill_2d_kernel(Box b, MV a, real_t v) {
-  LSMHIP_CELL(b);
-  a(i, j, k) = v;
-}
-
 __global__ void __launch_bounds__(256) advection_2d_kernel(Box fb, MV rhs, CV px, CV py, CV vx, CV vy) {
   LSMHIP_CELL(fb);
   rhs(i, j, k) = rhs(i, j, k) - (vx(i, j, k) * px(i, j, k) + vy(i, j, k) * py(i, j, k));
@@ -359,17 +354,20 @@ struct Region2 {
   real_t eps, dA;
   int sign;
   __device__ double operator()(int i, int j, int k) const {
-    const real_t p = phi(i, j, k), q = p / eps;
+    // q = phi / eps only inside the smoothing band, where it is used
+    const real_t p = phi(i, j, k);
     const real_t f = F.p ? F(i, j, k) : real_t(1);
     if (sign < 0) {
       if (p < -eps) return (double)(F.p ? f * dA : dA);
       if (p < eps) {
+        const real_t q = p / eps;
         const real_t h = real_t(0.5) * (real_t(1) - q - real_t(kOneOverPi) * sin(real_t(kPi) * q));
         return (double)(F.p ? h * f * dA : h * dA);
       }
     } else {
       if (p > eps) return (double)(F.p ? f * dA : dA);
       if (p > -eps) {
+        const real_t q = p / eps;
         const real_t h = real_t(0.5) * (real_t(1) + q + real_t(kOneOverPi) * sin(real_t(kPi) * q));
         return (double)(F.p ? h * f * dA : h * dA);
       }
@@ -609,8 +607,8 @@ LSMHIP_EXPORT void lsm2dzerooutlevelseteqnrhs_(real_t* rhs, LSMHIP_BOXP2(rg)) {
   const char* who = "lsm2dzerooutlevelseteqnrhs_";
   Box rg = LSMHIP_BOX2(rg);
   if (rg.empty() || !check_size(rg, who)) return;
-  hipLaunchKernelGGL(fill_2d_kernel, grid_for(rg), block2d(), 0, stream(), rg, view(rhs, rg), real_t(0));
-  finish(who);
+  // the whole ghost box, as the reference zeroes it: rg.volume() contiguous scalars from rhs
+  if (fill_contiguous(rhs, rg.volume(), real_t(0), who)) finish(who);
 }
 LSMHIP_EXPORT void lsm2daddadvectiontermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg), const real_t* px, const real_t* py,
                                                   LSMHIP_BOXP2(g), const real_t* vx, const real_t* vy,

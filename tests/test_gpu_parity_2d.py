@@ -53,6 +53,19 @@ def _cmp(hip, sym, c):
         H.assert_same(a, b, sym, skip=("D1", "D2", "D3"))
 
 
+REDUCTIONS_2D = sorted(SUMS_2D) + ["lsm2dmaxnormdiff_", "lsm2dcomputestableadvectiondt_",
+                                   "lsm2dcomputestablenormalveldt_", "lsm2dcomputestableconstnormalveldt_"]
+
+
+@pytest.mark.parametrize("sym", REDUCTIONS_2D)
+def test_reductions_2d_on_long_rows(hip, sym):
+    """Rows of 300 cells (one full 4 x 64 pass of a wave and a partial one),
+    5 rows, against the oracle."""
+    c = H.grid_case2d(7, gb=(-3, 302, -3, 7), fb=(0, 299, 0, 4))
+    prepare2d(c, sym)
+    _cmp(hip, sym, c)
+
+
 @pytest.mark.parametrize("sym", WHOLE_GRID_2D)
 @pytest.mark.parametrize("seed", [1, 2])
 def test_whole_grid_2d_bitwise(hip, sym, seed):

@@ -39,6 +39,50 @@ def test_whole_grid_bitwise(hip, sym, seed):
     _cmp(hip, sym, c)
 
 
+REDUCTIONS = sorted(SUMS) + ["lsm3dmaxnormdiff_", "lsm3dcomputestableadvectiondt_",
+                             "lsm3dcomputestablenormalveldt_", "lsm3dcomputestableconstnormalveldt_"]
+
+
+@pytest.mark.parametrize("sym", REDUCTIONS)
+def test_reductions_on_long_rows(hip, sym):
+    """The reductions visit a box one row per wave, 4 x 64 cells per pass:
+    rows of 300 cells (one full pass and a partial one) on 5 x 3 rows, fewer
+    rows than a block has waves in the last block, against the oracle."""
+    c = H.grid_case(7, gb=(-3, 302, -3, 7, -3, 5), fb=(0, 299, 0, 4, 0, 2))
+    _prepare(c, sym)
+    _cmp(hip, sym, c)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("offset", [0, 1, 2, 3])
+@pytest.mark.parametrize("gb", [(-3, 20, -3, 16, -3, 12), (0, 6, 0, 2, 0, 0), (0, 0, 0, 0, 0, 0),
+                                (-3, 300, -2, 40, -1, 30)])
+def test_zero_out_whole_ghost_box_at_any_alignment(prec, offset, gb):
+    """ZeroOut clears exactly the rhs ghost box (its volume of contiguous
+    scalars from the pointer), 2D and 3D, whatever the pointer's 16-byte
+    alignment; the guard cells either side keep their values."""
+    import torch
+    import lsmlib_amd
+    lib = lsmlib_amd.load(prec)
+    lib.set_stream(0, True)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    vol = (gb[1] - gb[0] + 1) * (gb[3] - gb[2] + 1) * (gb[5] - gb[4] + 1)
+    buf = torch.full((vol + 8,), 7.5, dtype=dt, device="cuda")
+    ptr = buf.data_ptr() + offset * buf.element_size()
+    lib.api.lsm3dzerooutlevelseteqnrhs(ptr, gb)
+    lib.check()
+    h = buf.cpu().numpy()
+    assert np.all(h[offset:offset + vol] == 0.0)
+    assert np.all(h[:offset] == 7.5) and np.all(h[offset + vol:] == 7.5)
+    if gb[4] == gb[5]:
+        buf.fill_(7.5)
+        lib.api.lsm2dzerooutlevelseteqnrhs(ptr, gb[:4])
+        lib.check()
+        h = buf.cpu().numpy()
+        assert np.all(h[offset:offset + vol] == 0.0)
+        assert np.all(h[:offset] == 7.5) and np.all(h[offset + vol:] == 7.5)
+
+
 @pytest.mark.parametrize("sym", QUIRKY_BOUNDS)
 def test_quirky_loop_bounds(hip, sym):
     c = H.grid_case(3, mixed_boxes=False)
