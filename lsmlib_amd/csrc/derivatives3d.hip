@@ -35,12 +35,15 @@ __global__ void __launch_bounds__(256) hj_plus_minus_kernel(Box fb, Axes3 plus, 
                                                             real_t iz) {
   LSMHIP_CELL(fb);
   const real_t inv[3] = {ix, iy, iz};
+  // All three lines are loaded before the first store: an output may alias phi
+  // as far as the compiler knows, so a load after a store would wait for it.
+  real_t f[3][7];
+#pragma unroll
+  for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f[d]);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    real_t f[7];
-    load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f);
     real_t p, m;
-    stencil::plus_minus<ORDER>(f, inv[d], p, m);
+    stencil::plus_minus<ORDER>(f[d], inv[d], p, m);
     plus.o[d](i, j, k) = p;
     minus.o[d](i, j, k) = m;
   }
@@ -53,35 +56,37 @@ __global__ void __launch_bounds__(256) hj_upwind_kernel(Box fb, Axes3 out, View<
   LSMHIP_CELL(fb);
   const real_t inv[3] = {ix, iy, iz};
   const real_t vel[3] = {vx(i, j, k), vy(i, j, k), vz(i, j, k)};
+  // Every line is loaded (inside phi's checked box) before the first store,
+  // also where the speed below zero_tol leaves it unused.
+  real_t f[3][7];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    real_t r;
-    if (fabs(vel[d]) < kZeroTol) {
-      r = real_t(0);
-    } else {
-      real_t f[7];
-      load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f);
-      r = stencil::upwind<ORDER>(f, inv[d], vel[d] > real_t(0));
-    }
-    out.o[d](i, j, k) = r;
-  }
+  for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f[d]);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    out.o[d](i, j, k) = fabs(vel[d]) < kZeroTol ? real_t(0) : stencil::upwind<ORDER>(f[d], inv[d], vel[d] > real_t(0));
 }
 
 __global__ void __launch_bounds__(256) central2_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
                                                        real_t fy, real_t fz) {
   LSMHIP_CELL(fb);
-  out.o[0](i, j, k) = (phi(i + 1, j, k) - phi(i - 1, j, k)) * fx;
-  out.o[1](i, j, k) = (phi(i, j + 1, k) - phi(i, j - 1, k)) * fy;
-  out.o[2](i, j, k) = (phi(i, j, k + 1) - phi(i, j, k - 1)) * fz;
+  // loads first: a load after a store to a possibly aliasing output would wait for it
+  const real_t xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, j + 1, k), ym = phi(i, j - 1, k),
+               zp = phi(i, j, k + 1), zm = phi(i, j, k - 1);
+  out.o[0](i, j, k) = (xp - xm) * fx;
+  out.o[1](i, j, k) = (yp - ym) * fy;
+  out.o[2](i, j, k) = (zp - zm) * fz;
 }
 
 __global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
                                                        real_t fy, real_t fz) {
   LSMHIP_CELL(fb);
   const real_t e = real_t(8);
-  out.o[0](i, j, k) = (-phi(i + 2, j, k) + e * phi(i + 1, j, k) + phi(i - 2, j, k) - e * phi(i - 1, j, k)) * fx;
-  out.o[1](i, j, k) = (-phi(i, j + 2, k) + e * phi(i, j + 1, k) + phi(i, j - 2, k) - e * phi(i, j - 1, k)) * fy;
-  out.o[2](i, j, k) = (-phi(i, j, k + 2) + e * phi(i, j, k + 1) + phi(i, j, k - 2) - e * phi(i, j, k - 1)) * fz;
+  real_t f[3][7];  // loads first, as in central2_kernel
+#pragma unroll
+  for (int d = 0; d < 3; d++) load_line<2>(phi, i, j, k, d, f[d]);
+  const real_t r[3] = {fx, fy, fz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) out.o[d](i, j, k) = (-f[d][5] + e * f[d][4] + f[d][1] - e * f[d][2]) * r[d];
 }
 
 __global__ void __launch_bounds__(256) laplacian2_kernel(Box fb, View<real_t> out, View<const real_t> phi,
@@ -95,8 +100,14 @@ __global__ void __launch_bounds__(256) laplacian2_kernel(Box fb, View<real_t> ou
 
 __global__ void __launch_bounds__(256) average_grad_kernel(Box fb, Axes3 out, Axes3 plus, Axes3 minus) {
   LSMHIP_CELL(fb);
+  real_t p[3], m[3];  // loads first, as in central2_kernel
 #pragma unroll
-  for (int d = 0; d < 3; d++) out.o[d](i, j, k) = real_t(0.5) * (plus.o[d](i, j, k) + minus.o[d](i, j, k));
+  for (int d = 0; d < 3; d++) {
+    p[d] = plus.o[d](i, j, k);
+    m[d] = minus.o[d](i, j, k);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) out.o[d](i, j, k) = real_t(0.5) * (p[d] + m[d]);
 }
 
 __global__ void __launch_bounds__(256) grad_mag_kernel(Box fb, Axes3 g, View<real_t> mag) {

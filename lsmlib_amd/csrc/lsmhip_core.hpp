@@ -109,17 +109,28 @@ int take_device_flag(const char* who);
 bool check_box(const Box& outer, const Box& inner, const char* who, const char* what);
 bool check_size(const Box& b, const char* who);
 
-// Launch shape for kernels that map one thread to one cell of a box.
-constexpr int kBX = 64, kBY = 4;
+// Launch shape for kernels that map one thread to one cell of a box: kBX x kBY
+// tiles of (i, j) in one plane k.  Workgroups are dealt to the kXcds XCDs
+// round robin in flattened-id order, and each XCD has its own L2.  The grid's x
+// extent is kXcds x (tiles along i), so block (bx, by, bz) runs on XCD bx % kXcds
+// whatever by and bz are; the macro gives XCD x the tiles of rows
+// [x * gridDim.y, (x + 1) * gridDim.y) of tiles along j, in every plane.  Tiles
+// that are neighbours along i, along j (inside a slab) and along k then share
+// one L2, so the cache lines at tile edges (rows start inside a line: fill
+// boxes sit three cells into the ghost box) and the stencils' neighbour rows
+// and planes are fetched once per XCD instead of once per neighbouring tile.
+// Tiles past the box (the last slab is partly empty) return at once.
+constexpr int kBX = 64, kBY = 4, kXcds = 8;
 inline dim3 grid_for(const Box& b) {
-  return dim3((unsigned)((b.nx() + kBX - 1) / kBX), (unsigned)((b.ny() + kBY - 1) / kBY), (unsigned)b.nz());
+  const unsigned ntx = (unsigned)((b.nx() + kBX - 1) / kBX), nty = (unsigned)((b.ny() + kBY - 1) / kBY);
+  return dim3(kXcds * ntx, (nty + kXcds - 1) / kXcds, (unsigned)b.nz());
 }
 inline dim3 block2d() { return dim3(kBX, kBY, 1); }
 
-#define LSMHIP_CELL(b)                              \
-  const int i = (b).ilo + blockIdx.x * kBX + threadIdx.x; \
-  const int j = (b).jlo + blockIdx.y * kBY + threadIdx.y; \
-  const int k = (b).klo + blockIdx.z;               \
+#define LSMHIP_CELL(b)                                                                            \
+  const int i = (b).ilo + (int)(blockIdx.x / kXcds) * kBX + (int)threadIdx.x;                    \
+  const int j = (b).jlo + (int)((blockIdx.x % kXcds) * gridDim.y + blockIdx.y) * kBY + (int)threadIdx.y; \
+  const int k = (b).klo + (int)blockIdx.z;                                                        \
   if (i > (b).ihi || j > (b).jhi) return
 
 // Exported-signature helpers: six box bounds passed by pointer.

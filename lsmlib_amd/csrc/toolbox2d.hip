@@ -59,13 +59,15 @@ template <int ORDER>
 __global__ void __launch_bounds__(256) hj_pm2_kernel(Box fb, MV xp, MV yp, MV xm, MV ym, CV phi, real_t ix,
                                                      real_t iy) {
   LSMHIP_CELL(fb);
-  real_t f[7], p, m;
+  // Both lines are loaded before the first store: an output may alias phi as
+  // far as the compiler knows, so a load after a store would wait for it.
+  real_t f[7], g[7], p, m;
   line2<reach(ORDER)>(phi, i, j, 0, f);
+  line2<reach(ORDER)>(phi, i, j, 1, g);
   stencil::plus_minus<ORDER>(f, ix, p, m);
   xp(i, j, k) = p;
   xm(i, j, k) = m;
-  line2<reach(ORDER)>(phi, i, j, 1, f);
-  stencil::plus_minus<ORDER>(f, iy, p, m);
+  stencil::plus_minus<ORDER>(g, iy, p, m);
   yp(i, j, k) = p;
   ym(i, j, k) = m;
 }
@@ -75,29 +77,31 @@ __global__ void __launch_bounds__(256) hj_up2_kernel(Box fb, MV ox, MV oy, CV ph
                                                      real_t iy) {
   LSMHIP_CELL(fb);
   const real_t vel[2] = {vx(i, j, k), vy(i, j, k)}, inv[2] = {ix, iy};
+  // Both lines are loaded (inside phi's checked box) before the first store,
+  // also where the speed below zero_tol leaves one unused.
+  real_t f[2][7];
 #pragma unroll
-  for (int d = 0; d < 2; d++) {
-    real_t r = real_t(0);
-    if (!(fabs(vel[d]) < kZeroTol)) {
-      real_t f[7];
-      line2<reach(ORDER)>(phi, i, j, d, f);
-      r = stencil::upwind<ORDER>(f, inv[d], vel[d] > real_t(0));
-    }
-    (d == 0 ? ox : oy)(i, j, k) = r;
-  }
+  for (int d = 0; d < 2; d++) line2<reach(ORDER)>(phi, i, j, d, f[d]);
+#pragma unroll
+  for (int d = 0; d < 2; d++)
+    (d == 0 ? ox : oy)(i, j, k) =
+        fabs(vel[d]) < kZeroTol ? real_t(0) : stencil::upwind<ORDER>(f[d], inv[d], vel[d] > real_t(0));
 }
 
 __global__ void __launch_bounds__(256) central2_2d_kernel(Box fb, MV ox, MV oy, CV phi, real_t fx, real_t fy) {
   LSMHIP_CELL(fb);
-  ox(i, j, k) = (phi(i + 1, j, k) - phi(i - 1, j, k)) * fx;
-  oy(i, j, k) = (phi(i, j + 1, k) - phi(i, j - 1, k)) * fy;
+  const real_t xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, j + 1, k), ym = phi(i, j - 1, k);
+  ox(i, j, k) = (xp - xm) * fx;
+  oy(i, j, k) = (yp - ym) * fy;
 }
 
 __global__ void __launch_bounds__(256) central4_2d_kernel(Box fb, MV ox, MV oy, CV phi, real_t fx, real_t fy) {
   LSMHIP_CELL(fb);
   const real_t e = real_t(8);
-  ox(i, j, k) = (-phi(i + 2, j, k) + e * phi(i + 1, j, k) + phi(i - 2, j, k) - e * phi(i - 1, j, k)) * fx;
-  oy(i, j, k) = (-phi(i, j + 2, k) + e * phi(i, j + 1, k) + phi(i, j - 2, k) - e * phi(i, j - 1, k)) * fy;
+  const real_t a2 = phi(i + 2, j, k), a1 = phi(i + 1, j, k), b2 = phi(i - 2, j, k), b1 = phi(i - 1, j, k);
+  const real_t c2 = phi(i, j + 2, k), c1 = phi(i, j + 1, k), d2 = phi(i, j - 2, k), d1 = phi(i, j - 1, k);
+  ox(i, j, k) = (-a2 + e * a1 + b2 - e * b1) * fx;
+  oy(i, j, k) = (-c2 + e * c1 + d2 - e * d1) * fy;
 }
 
 __global__ void __launch_bounds__(256) laplacian2_2d_kernel(Box fb, MV out, CV phi, real_t ax, real_t ay) {
@@ -109,8 +113,9 @@ __global__ void __launch_bounds__(256) laplacian2_2d_kernel(Box fb, MV out, CV p
 
 __global__ void __launch_bounds__(256) average_grad_2d_kernel(Box fb, MV ox, MV oy, CV xp, CV yp, CV xm, CV ym) {
   LSMHIP_CELL(fb);
-  ox(i, j, k) = real_t(0.5) * (xp(i, j, k) + xm(i, j, k));
-  oy(i, j, k) = real_t(0.5) * (yp(i, j, k) + ym(i, j, k));
+  const real_t a = xp(i, j, k), b = xm(i, j, k), c = yp(i, j, k), d = ym(i, j, k);
+  ox(i, j, k) = real_t(0.5) * (a + b);
+  oy(i, j, k) = real_t(0.5) * (c + d);
 }
 
 // lsm_spatial_derivatives2d.f.in:2395-2448: both cut-offs are zero_tol in 2D.
