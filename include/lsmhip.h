@@ -125,6 +125,34 @@ int lsmhip_weno5_local_normal_dt(LSMLIB_REAL *dt, const LSMLIB_REAL *phi, LSMHIP
                                  const int *index_z, int nlo_index0, int nhi_index0,
                                  const unsigned char *narrow_band, LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb,
                                  unsigned char mark_D1);
+/* lsmhip_weno5_local_normal_dt, and the RHS the first stage of the same chain
+ * would form from phi (ZERO_OUT_LOCAL -> ADD_[CONST_]NORMAL_VEL_TERM_LOCAL ->
+ * MULTIPLY_CUT_OFF_LSE_RHS_LOCAL(phi, beta, gamma; none when gamma <= 0)) at
+ * each level-0 point, stored compactly: rhs[t] for entry nlo_index0 + t,
+ * rhs holding nhi_index0 - nlo_index0 + 1 reals on the device.  Entries of
+ * points the stage skips (mark above mark_fb, or outside the ghost boxes) are
+ * left as they are; a point whose stencil reaches outside the band is reported
+ * as in lsmhip_weno5_local_normal_dt and its entry marked to be skipped.  The
+ * CFL pass already computes that point's derivatives, so the first stage need
+ * not compute them again (lsmhip_local_stage_from_rhs). */
+int lsmhip_weno5_local_normal_dt_rhs(LSMLIB_REAL *dt, LSMLIB_REAL *rhs, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb),
+                                     LSMLIB_REAL vn, const LSMLIB_REAL *vn_field, LSMLIB_REAL dx, LSMLIB_REAL dy,
+                                     LSMLIB_REAL dz, LSMLIB_REAL cfl_number, const int *index_x, const int *index_y,
+                                     const int *index_z, int nlo_index0, int nhi_index0,
+                                     const unsigned char *narrow_band, LSMHIP_BOX_ARGS(nb_gb),
+                                     unsigned char mark_fb, unsigned char mark_D1, LSMLIB_REAL beta,
+                                     LSMLIB_REAL gamma);
+/* The stage of lsmhip_weno5_local_stage_normal (same stages, points, copies
+ * over nlo_index0..nhi_copy and RK combination) with the RHS taken from rhs as
+ * lsmhip_weno5_local_normal_dt_rhs stored it instead of computed: with that
+ * rhs (of phi_stage) the result is bitwise the fused stage's, and so the
+ * chain's in FP64.  An entry marked to be skipped is skipped and reported
+ * (LSMHIP_ERR_INDEX).  Returns 0 or -1. */
+int lsmhip_local_stage_from_rhs(LSMLIB_REAL *phi_out, const LSMLIB_REAL *phi_stage, const LSMLIB_REAL *phi_n,
+                                LSMHIP_BOX_ARGS(gb), int stage, const LSMLIB_REAL *rhs, LSMLIB_REAL dt,
+                                const int *index_x, const int *index_y, const int *index_z, int nlo_index0,
+                                int nhi_index0, int nhi_copy, const unsigned char *narrow_band,
+                                LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb);
 
 /* ---- reverse mode of the normal-motion stage (for gradient-based inverse problems) ----
  * Reverse of lsmhip_weno5_stage_normal for stage 1, 2, 3 (TVD-RK3) or 22

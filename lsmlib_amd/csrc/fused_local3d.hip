@@ -18,6 +18,8 @@
 // would be a stale difference in the chain; the kernel skips such a point and
 // reports it (LSMHIP_ERR_INDEX) instead.  With that, FP64 results are bitwise
 // the chain's; FP32 uses the rearranged WENO5 of the whole-grid fused stages.
+#include <type_traits>
+
 #include "local.hpp"
 #include "stencil.hpp"
 
@@ -123,6 +125,34 @@ __device__ __forceinline__ bool band_point(const LocalStage& a, long long t, int
   return true;
 }
 
+// The RHS of one stage at a level-0 point: the Godunov normal-speed term
+// (ADD_[CONST_]NORMAL_VEL_TERM_LOCAL on a zeroed RHS), times the cut-off of
+// MULTIPLY_CUT_OFF_LSE_RHS_LOCAL when one is set, in the chain's operation order.
+__device__ __forceinline__ real_t stage_rhs(const LocalStage& a, real_t v, const real_t g[6], real_t us) {
+  real_t r = real_t(0);
+  if (fabs(v) >= kZeroTol) r = r - v * local_sqrt(stencil::godunov_sq(v > real_t(0), g[0], g[1], g[2], g[3], g[4], g[5]));
+  if (a.cutoff) {
+    const real_t ab = fabs(us);
+    real_t cut;
+    if (ab <= a.beta) {
+      cut = real_t(1);
+    } else if (ab <= a.gamma) {
+      const real_t q = ab - a.gamma;
+      cut = (q * q * (real_t(2) * ab + a.c1)) / a.c2;
+    } else {
+      cut = real_t(0);
+    }
+    r = cut * r;
+  }
+  return r;
+}
+
+// Stored in rhs[t] where the stencil may not be evaluated: a signalling-NaN
+// bit pattern, which no arithmetic produces (the hardware's NaNs are quiet),
+// written and compared as an integer, never as a real.
+typedef std::conditional<sizeof(real_t) == 8, unsigned long long, unsigned int>::type rhs_bits_t;
+constexpr rhs_bits_t kSkipBits = sizeof(real_t) == 8 ? (rhs_bits_t)0x7ff40000000051abULL : (rhs_bits_t)0x7fa051abU;
+
 template <int STAGE>
 __device__ __forceinline__ real_t rk_combine(real_t uc, real_t us, real_t dt, real_t r) {
   if (STAGE == 1) return uc + dt * r;
@@ -164,37 +194,61 @@ __global__ void __launch_bounds__(kT) local_normal_stage_kernel(LocalStage a, in
   const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
   real_t g[6];  // xp, yp, zp, xm, ym, zm
   if (!derivatives_local(a, i, j, k, p, g, flag)) return;
-  real_t r = real_t(0);
-  if (fabs(v) >= kZeroTol) r = r - v * local_sqrt(stencil::godunov_sq(v > real_t(0), g[0], g[1], g[2], g[3], g[4], g[5]));
   const real_t us = a.us[p];
-  if (a.cutoff) {
-    const real_t ab = fabs(us);
-    real_t cut;
-    if (ab <= a.beta) {
-      cut = real_t(1);
-    } else if (ab <= a.gamma) {
-      const real_t q = ab - a.gamma;
-      cut = (q * q * (real_t(2) * ab + a.c1)) / a.c2;
-    } else {
-      cut = real_t(0);
-    }
-    r = cut * r;
+  a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, stage_rhs(a, v, g, us));
+}
+
+// Stage from a stored RHS (lsmhip_weno5_local_normal_dt_rhs): the same points,
+// copies and RK combination as local_normal_stage_kernel, with rhs[t] in place
+// of the derivatives; an entry holding the skip pattern is skipped and flagged.
+template <int STAGE>
+__global__ void __launch_bounds__(kT) local_stage_from_rhs_kernel(LocalStage a, const real_t* rhs, int* flag) {
+  const long long t = (long long)blockIdx.x * kT + threadIdx.x;
+  if (t > (long long)a.L.hi - a.L.lo) {
+    pass_through<STAGE>(a, t, flag);
+    return;
   }
+  int i, j, k;
+  long long p;
+  if (!band_point(a, t, i, j, k, p, flag)) {
+    if (a.fbm.b.has(i, j, k)) pass_through<STAGE>(a, t, flag);
+    return;
+  }
+  const rhs_bits_t rb = reinterpret_cast<const rhs_bits_t*>(rhs)[t];
+  if (rb == kSkipBits) {
+    *flag = 1;
+    return;
+  }
+  real_t r;
+  __builtin_memcpy(&r, &rb, sizeof(r));
+  const real_t us = a.us[p];
+  const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
   a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, r);
 }
 
 // CFL of normal motion at one level-0 point: lsm3dComputeStableNormalVelDtLOCAL
 // with a speed field, the whole-grid constant-speed formula otherwise.
+// With rhs set it also stores the stage RHS of the point at rhs[t] (the skip
+// pattern where the stencil may not be evaluated), for a first stage from
+// lsmhip_local_stage_from_rhs.
 struct LocalDtOp {
   LocalStage a;
   real_t mdx2;
   int* flag;
-  __device__ bool operator()(int i, int j, int k, double& r) const {
-    if (!a.gb.has(i, j, k)) return false;
+  real_t* rhs;
+  __device__ bool operator()(long long t, int i, int j, int k, double& r) const {
+    if (!a.gb.has(i, j, k)) {
+      if (rhs) reinterpret_cast<rhs_bits_t*>(rhs)[t] = kSkipBits;
+      return false;
+    }
     const long long p =
         (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
     real_t g[6];
-    if (!derivatives_local(a, i, j, k, p, g, flag)) return false;
+    if (!derivatives_local(a, i, j, k, p, g, flag)) {
+      if (rhs) reinterpret_cast<rhs_bits_t*>(rhs)[t] = kSkipBits;
+      return false;
+    }
+    if (rhs) rhs[t] = stage_rhs(a, a.vnf ? a.vnf[p] : a.vn, g, a.us[p]);
     const real_t av = a.vnf ? fabs(a.vnf[p]) : fabs(a.vn);
     real_t n;
     const real_t h = stencil::normal_h_over_dx(av, g[0], g[1], g[2], g[3], g[4], g[5], mdx2, a.inv[0], a.inv[1],
@@ -227,6 +281,22 @@ LocalStage make_stage(real_t* out, const real_t* us, const real_t* uc, const Box
   return a;
 }
 
+// The cut-off of MULTIPLY_CUT_OFF_LSE_RHS_LOCAL (none when gamma <= 0).
+void set_cutoff(LocalStage& a, real_t beta, real_t gamma) {
+  a.cutoff = gamma > real_t(0);
+  a.beta = beta;
+  a.gamma = gamma;
+  a.c1 = gamma - real_t(3) * beta;
+  const real_t c2 = gamma - beta;
+  a.c2 = c2 * c2 * c2;
+}
+
+bool check_stage(const char* who, int stage) {
+  if (stage == 1 || stage == 2 || stage == 3 || stage == 22) return true;
+  set_error(kErrArg, "%s: stage %d is not 1, 2, 3 or 22", who, stage);
+  return false;
+}
+
 }  // namespace
 
 LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LSMLIB_REAL* phi_stage,
@@ -240,10 +310,7 @@ LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LS
   const char* who = "lsmhip_weno5_local_stage_normal";
   const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
   const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
-  if (stage != 1 && stage != 2 && stage != 3 && stage != 22) {
-    set_error(kErrArg, "%s: stage %d is not 1, 2, 3 or 22", who, stage);
-    return -1;
-  }
+  if (!check_stage(who, stage)) return -1;
   if (stage != 1 && !phi_n) {
     set_error(kErrArg, "%s: stage %d needs phi_n", who, stage);
     return -1;
@@ -259,12 +326,7 @@ LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LS
                             narrow_band, nbb, mark_fb, mark_D1, dx, dy, dz, vn, vn_field);
   a.ncopy = nhi_copy >= nlo_index0 ? nhi_copy - nlo_index0 + 1 : 0;
   a.dt = dt;
-  a.cutoff = gamma > real_t(0);
-  a.beta = beta;
-  a.gamma = gamma;
-  a.c1 = gamma - real_t(3) * beta;
-  real_t c2 = gamma - beta;
-  a.c2 = c2 * c2 * c2;
+  set_cutoff(a, beta, gamma);
   int* flag = device_flag();
   if (!flag) return -1;
   const dim3 grid(xcd_grid((long long)hi - nlo_index0 + 1));
@@ -280,15 +342,11 @@ LSMHIP_EXPORT int lsmhip_weno5_local_stage_normal(LSMLIB_REAL* phi_out, const LS
   return lsmhip_get_last_error() == 0 ? 0 : -1;
 }
 
-LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt(LSMLIB_REAL* dt, const LSMLIB_REAL* phi, LSMHIP_BOX_ARGS(gb),
-                                               LSMLIB_REAL vn, const LSMLIB_REAL* vn_field, LSMLIB_REAL dx,
-                                               LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL cfl_number,
-                                               const int* index_x, const int* index_y, const int* index_z,
-                                               int nlo_index0, int nhi_index0, const unsigned char* narrow_band,
-                                               LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb, unsigned char mark_D1) {
-  const char* who = "lsmhip_weno5_local_normal_dt";
-  const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
-  const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
+static int local_normal_dt(const char* who, real_t* dt, real_t* rhs, const real_t* phi, const Box& gb, real_t vn,
+                           const real_t* vn_field, real_t dx, real_t dy, real_t dz, real_t cfl_number,
+                           const int* index_x, const int* index_y, const int* index_z, int nlo_index0,
+                           int nhi_index0, const unsigned char* narrow_band, const Box& nbb, unsigned char mark_fb,
+                           unsigned char mark_D1, real_t beta, real_t gamma) {
   if (!dt || !phi || !index_x || !index_y || !index_z || !narrow_band) {
     set_error(kErrArg, "%s: a required array is NULL", who);
     return -1;
@@ -298,12 +356,86 @@ LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt(LSMLIB_REAL* dt, const LSMLIB_REA
   if (!flag) return -1;
   LocalStage a = make_stage(nullptr, phi, nullptr, gb, index_x, index_y, index_z, nlo_index0, nhi_index0,
                             narrow_band, nbb, mark_fb, mark_D1, dx, dy, dz, vn, vn_field);
+  set_cutoff(a, beta, gamma);
   real_t h = dx > dy ? dx : dy;
   h = h > dz ? h : dz;
   double m = -1.0;
   bool any = false;
-  if (!points_max(who, a.L, a.fbm, LocalDtOp{a, h * h, flag}, &m, &any)) return -1;
+  if (!points_max(who, a.L, a.fbm, LocalDtOp{a, h * h, flag, rhs}, &m, &any)) return -1;
   if (!any) m = -1.0;
   *dt = cfl_number / (real_t(m) + kTiny);
+  return lsmhip_get_last_error() == 0 ? 0 : -1;
+}
+
+LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt(LSMLIB_REAL* dt, const LSMLIB_REAL* phi, LSMHIP_BOX_ARGS(gb),
+                                               LSMLIB_REAL vn, const LSMLIB_REAL* vn_field, LSMLIB_REAL dx,
+                                               LSMLIB_REAL dy, LSMLIB_REAL dz, LSMLIB_REAL cfl_number,
+                                               const int* index_x, const int* index_y, const int* index_z,
+                                               int nlo_index0, int nhi_index0, const unsigned char* narrow_band,
+                                               LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb, unsigned char mark_D1) {
+  return local_normal_dt("lsmhip_weno5_local_normal_dt", dt, nullptr, phi,
+                         box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb), vn, vn_field, dx, dy, dz, cfl_number,
+                         index_x, index_y, index_z, nlo_index0, nhi_index0, narrow_band,
+                         box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb), mark_fb, mark_D1,
+                         real_t(0), real_t(0));
+}
+
+LSMHIP_EXPORT int lsmhip_weno5_local_normal_dt_rhs(LSMLIB_REAL* dt, LSMLIB_REAL* rhs, const LSMLIB_REAL* phi,
+                                                   LSMHIP_BOX_ARGS(gb), LSMLIB_REAL vn, const LSMLIB_REAL* vn_field,
+                                                   LSMLIB_REAL dx, LSMLIB_REAL dy, LSMLIB_REAL dz,
+                                                   LSMLIB_REAL cfl_number, const int* index_x, const int* index_y,
+                                                   const int* index_z, int nlo_index0, int nhi_index0,
+                                                   const unsigned char* narrow_band, LSMHIP_BOX_ARGS(nb_gb),
+                                                   unsigned char mark_fb, unsigned char mark_D1, LSMLIB_REAL beta,
+                                                   LSMLIB_REAL gamma) {
+  const char* who = "lsmhip_weno5_local_normal_dt_rhs";
+  if (!rhs) {
+    set_error(kErrArg, "%s: a required array is NULL", who);
+    return -1;
+  }
+  return local_normal_dt(who, dt, rhs, phi, box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb), vn, vn_field,
+                         dx, dy, dz, cfl_number, index_x, index_y, index_z, nlo_index0, nhi_index0, narrow_band,
+                         box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb), mark_fb, mark_D1,
+                         beta, gamma);
+}
+
+LSMHIP_EXPORT int lsmhip_local_stage_from_rhs(LSMLIB_REAL* phi_out, const LSMLIB_REAL* phi_stage,
+                                              const LSMLIB_REAL* phi_n, LSMHIP_BOX_ARGS(gb), int stage,
+                                              const LSMLIB_REAL* rhs, LSMLIB_REAL dt, const int* index_x,
+                                              const int* index_y, const int* index_z, int nlo_index0,
+                                              int nhi_index0, int nhi_copy, const unsigned char* narrow_band,
+                                              LSMHIP_BOX_ARGS(nb_gb), unsigned char mark_fb) {
+  const char* who = "lsmhip_local_stage_from_rhs";
+  const Box gb = box_of(ilo_gb, ihi_gb, jlo_gb, jhi_gb, klo_gb, khi_gb);
+  const Box nbb = box_of(ilo_nb_gb, ihi_nb_gb, jlo_nb_gb, jhi_nb_gb, klo_nb_gb, khi_nb_gb);
+  if (!check_stage(who, stage)) return -1;
+  if (stage != 1 && !phi_n) {
+    set_error(kErrArg, "%s: stage %d needs phi_n", who, stage);
+    return -1;
+  }
+  if (!phi_out || !phi_stage || !rhs || !index_x || !index_y || !index_z || !narrow_band) {
+    set_error(kErrArg, "%s: a required array is NULL", who);
+    return -1;
+  }
+  if (!check_size(gb, who) || !check_size(nbb, who)) return -1;
+  const int hi = nhi_copy > nhi_index0 ? nhi_copy : nhi_index0;
+  if (hi < nlo_index0) return 0;
+  LocalStage a = make_stage(phi_out, phi_stage, phi_n, gb, index_x, index_y, index_z, nlo_index0, nhi_index0,
+                            narrow_band, nbb, mark_fb, mark_fb, real_t(1), real_t(1), real_t(1), real_t(0), nullptr);
+  a.ncopy = nhi_copy >= nlo_index0 ? nhi_copy - nlo_index0 + 1 : 0;
+  a.dt = dt;
+  int* flag = device_flag();
+  if (!flag) return -1;
+  const long long n = (long long)hi - nlo_index0 + 1;
+  const dim3 grid((unsigned)((n + kT - 1) / kT));
+  switch (stage) {
+    case 1: hipLaunchKernelGGL(local_stage_from_rhs_kernel<1>, grid, dim3(kT), 0, stream(), a, rhs, flag); break;
+    case 2: hipLaunchKernelGGL(local_stage_from_rhs_kernel<2>, grid, dim3(kT), 0, stream(), a, rhs, flag); break;
+    case 3: hipLaunchKernelGGL(local_stage_from_rhs_kernel<3>, grid, dim3(kT), 0, stream(), a, rhs, flag); break;
+    default: hipLaunchKernelGGL(local_stage_from_rhs_kernel<22>, grid, dim3(kT), 0, stream(), a, rhs, flag); break;
+  }
+  if (!check_hip(hipGetLastError(), who)) return -1;
+  if (!finish(who)) return -1;
+  if (synchronous()) take_device_flag(who);
   return lsmhip_get_last_error() == 0 ? 0 : -1;
 }

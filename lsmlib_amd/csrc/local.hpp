@@ -151,6 +151,18 @@ void dn_local(const char* who, real_t* dn, const Box& dnb, const real_t* dm, con
 }
 
 // ----- reductions over index lists ----------------------------------------------------------------
+// An op of the list maxima is called as op(t, i, j, k, r) when it takes the
+// entry's position t in the list (0 for entry lo), else as op(i, j, k, r).
+template <class Op>
+__device__ __forceinline__ auto call_max_op(const Op& op, long long t, int i, int j, int k, double& r, int)
+    -> decltype(op(t, i, j, k, r)) {
+  return op(t, i, j, k, r);
+}
+template <class Op>
+__device__ __forceinline__ bool call_max_op(const Op& op, long long, int i, int j, int k, double& r, long) {
+  return op(i, j, k, r);
+}
+
 // Max over admissible points of non-negative values (-1 stands for "no
 // value": a point that is not admissible, or an op that declines it).
 template <class Op>
@@ -163,7 +175,7 @@ __global__ void __launch_bounds__(kT) points_max_kernel(IndexList L, Mask M, Op 
     int i, j, k;
     L.at(L.lo + (int)t, i, j, k);
     if (!M.b.has(i, j, k)) *flag = 1;
-    else if (M.at(i, j, k) <= M.mark && !op(i, j, k, m)) *flag = 1;
+    else if (M.at(i, j, k) <= M.mark && !call_max_op(op, t, i, j, k, m, 0)) *flag = 1;
   }
   m = reduce::wave_max(m);
   __shared__ double w[kT / 64];

@@ -558,9 +558,11 @@ class NarrowBandRun:
         -> ZERO_OUT_LOCAL -> ADD_CONST_NORMAL_VEL_TERM_LOCAL (V = 1)
         -> MULTIPLY_CUT_OFF_LSE_RHS_LOCAL(beta = width_inner, gamma = width)
         -> TVD_RK3_STAGEn_LOCAL -> signed linear extrapolation fill.
-    fused=True runs the same step through lsmhip_weno5_local_normal_dt and one
-    lsmhip_weno5_local_stage_normal launch per stage, the copies folded into
-    stages 1 and 2 (FP64: bitwise the same fields)."""
+    fused=True runs the same step through lsmhip_weno5_local_normal_dt_rhs
+    (the CFL step, which also stores the first stage's RHS per level-0 point
+    from the derivatives it computes), lsmhip_local_stage_from_rhs for stage 1
+    and lsmhip_weno5_local_stage_normal for stages 2 and 3, the copies folded
+    into stages 1 and 2 (FP64: bitwise the same fields)."""
 
     MARKS = (124, 125, 126, 127)  # fb, D2, D1, gb
     CFL = 0.5
@@ -601,6 +603,12 @@ class NarrowBandRun:
                                 [P, P, P] + box + [I, R, P, R, R, R, R, P, P, P, I, I, I, P] + box + [U8, U8, R, R])
             self._fdt = L.fn("lsmhip_weno5_local_normal_dt",
                              [ctypes.POINTER(R), P] + box + [R, P, R, R, R, R, P, P, P, I, I, P] + box + [U8, U8])
+            self._fdtr = L.fn("lsmhip_weno5_local_normal_dt_rhs",
+                              [ctypes.POINTER(R), P, P] + box + [R, P, R, R, R, R, P, P, P, I, I, P] + box +
+                              [U8, U8, R, R])
+            self._fsrhs = L.fn("lsmhip_local_stage_from_rhs",
+                               [P, P, P] + box + [I, P, R, P, P, P, I, I, I, P] + box + [U8])
+            self._rhs0 = None  # the first stage's RHS per level-0 entry
 
     def boxes(self):
         n, gb = self.grid.n, self.grid.gb
@@ -678,17 +686,24 @@ class NarrowBandRun:
         mfb, md1 = self.MARKS[0], self.MARKS[2]
         phi, (p1, p2) = self.phi, self.work
         gbp = s._gb()
+        n0 = hi0 - lo0 + 1
+        if self._rhs0 is None or self._rhs0.numel() < n0:
+            self._rhs0 = s.torch.empty(max(n0, 1), dtype=s.dtype, device=s.device)
         dt = s.real(0)
-        s._ok(self._fdt(ctypes.byref(dt), phi.data_ptr(), *gbp, 1.0, None, *g.dx, self.CFL, *ptrs, lo0, hi0,
-                        self.nb.data_ptr(), *gbp, mfb, md1))
+        s._ok(self._fdtr(ctypes.byref(dt), self._rhs0.data_ptr(), phi.data_ptr(), *gbp, 1.0, None, *g.dx, self.CFL,
+                         *ptrs, lo0, hi0, self.nb.data_ptr(), *gbp, mfb, md1, self.width_inner, self.width))
         dt = dt.value
         self.dts.append(dt)
         for st, (src, out) in enumerate(((phi, p1), (p1, p2), (p2, phi)), start=1):
             # stages 1 and 2 also carry phi^n into their outputs over levels
             # 0..3 (the drop-in sequence copies phi there before stage 1)
-            s._ok(self._fstage(out.data_ptr(), src.data_ptr(), phi.data_ptr(), *gbp, st, 1.0, None, dt, *g.dx,
-                               *ptrs, lo0, hi0, hi3 if st < 3 else -1, self.nb.data_ptr(), *gbp, mfb, md1,
-                               self.width_inner, self.width))
+            if st == 1:
+                s._ok(self._fsrhs(out.data_ptr(), src.data_ptr(), phi.data_ptr(), *gbp, st, self._rhs0.data_ptr(),
+                                  dt, *ptrs, lo0, hi0, hi3, self.nb.data_ptr(), *gbp, mfb))
+            else:
+                s._ok(self._fstage(out.data_ptr(), src.data_ptr(), phi.data_ptr(), *gbp, st, 1.0, None, dt, *g.dx,
+                                   *ptrs, lo0, hi0, hi3 if st < 3 else -1, self.nb.data_ptr(), *gbp, mfb, md1,
+                                   self.width_inner, self.width))
             s.fill_ghosts(out)
         s.lib.check()
         self.nsteps += 1
