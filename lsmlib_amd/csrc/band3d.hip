@@ -104,26 +104,39 @@ __global__ void __launch_bounds__(kT) level0_count_kernel(Level0 a, int* counts)
 }
 
 // Exclusive scan of n interleaved triples (one block); totals to tot[0..2].
+// Each thread sums a contiguous segment, the 1024 segment sums are scanned
+// in parallel (shuffles inside each wave, then the 16 wave totals), and each
+// thread rewrites its segment from its offset.
 __global__ void __launch_bounds__(1024) scan3_kernel(int* v, long long n, long long* tot) {
-  __shared__ long long part[3][1024];
+  __shared__ long long wsum[3][1024 / 64];
   const long long per = (n + 1023) / 1024;
   const long long lo = threadIdx.x * per, hi = min(n, lo + per);
   long long s[3] = {0, 0, 0};
   for (long long t = lo; t < hi; t++)
     for (int q = 0; q < 3; q++) s[q] += v[3 * t + q];
-  for (int q = 0; q < 3; q++) part[q][threadIdx.x] = s[q];
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    long long acc = 0;
-    for (int u = 0; u < 1024; u++) {
-      const long long x = part[threadIdx.x][u];
-      part[threadIdx.x][u] = acc;
-      acc += x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  long long inc[3];
+  for (int q = 0; q < 3; q++) {
+    long long x = s[q];
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    tot[threadIdx.x] = acc;
+    inc[q] = x;  // inclusive within the wave
   }
+  if (lane == 63)
+    for (int q = 0; q < 3; q++) wsum[q][w] = inc[q];
   __syncthreads();
-  long long run[3] = {part[0][threadIdx.x], part[1][threadIdx.x], part[2][threadIdx.x]};
+  long long run[3];
+  for (int q = 0; q < 3; q++) {
+    long long before = 0, all = 0;
+    for (int u = 0; u < 1024 / 64; u++) {
+      if (u < w) before += wsum[q][u];
+      all += wsum[q][u];
+    }
+    run[q] = before + inc[q] - s[q];
+    if (threadIdx.x == 0) tot[q] = all;
+  }
   for (long long t = lo; t < hi; t++)
     for (int q = 0; q < 3; q++) {
       const int x = v[3 * t + q];
@@ -145,17 +158,24 @@ __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* o
   __shared__ int lds[3][kT / 64];
   const long long base = (long long)blockIdx.x * kChunk;
   int run[3] = {offsets[3 * blockIdx.x], offsets[3 * blockIdx.x + 1], offsets[3 * blockIdx.x + 2]};
+  // (i, j, k) of the thread's first cell, then advanced by kT cells per pass:
+  // one division per thread instead of three per cell
+  const long long t0 = base + threadIdx.x;
+  int i = a.B.ilo + (int)(t0 % a.nx);
+  int j = a.B.jlo + (int)((t0 / a.nx) % a.ny);
+  int k = a.B.klo + (int)(t0 / ((long long)a.nx * a.ny));
+  const bool same = o.nbb.ilo == a.B.ilo && o.nbb.ihi == a.B.ihi && o.nbb.jlo == a.B.jlo && o.nbb.jhi == a.B.jhi &&
+                    o.nbb.klo == a.B.klo && o.nbb.khi == a.B.khi;
   for (int it = 0; it < kPer; it++) {
-    const long long t = base + it * kT + threadIdx.x;
+    const long long t = t0 + it * kT;
     bool f[3];
     a.flags(t, f);
     int pre[3], tot[3];
     block_prefix3(f, pre, tot, lds);
     if (t < a.n) {
-      const int i = a.B.ilo + (int)(t % a.nx);
-      const int j = a.B.jlo + (int)((t / a.nx) % a.ny);
-      const int k = a.B.klo + (int)(t / ((long long)a.nx * a.ny));
-      const int nbo = (i - o.nbb.ilo) + o.nbb.nx() * ((j - o.nbb.jlo) + o.nbb.ny() * (k - o.nbb.klo));
+      const long long nbo = same ? t
+                                 : (long long)(i - o.nbb.ilo) +
+                                       (long long)o.nbb.nx() * ((j - o.nbb.jlo) + (long long)o.nbb.ny() * (k - o.nbb.klo));
       o.nb[nbo] = f[0] ? 1 : 0;
       if (f[0]) {
         const int r = run[0] + pre[0];
@@ -167,6 +187,14 @@ __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* o
       }
     }
     for (int q = 0; q < 3; q++) run[q] += tot[q];
+    i += kT;
+    while (i > a.B.ihi) {
+      i -= a.nx;
+      if (++j > a.B.jhi) {
+        j = a.B.jlo;
+        ++k;
+      }
+    }
   }
 }
 
@@ -180,11 +208,15 @@ struct Grow {
   unsigned char* nb;
   Box nbb;
   int* claim;               // over nbb
+  unsigned char* win;       // per candidate: 1 where it won its cell (written by the count pass)
 };
 
+// NDIR (6 in 3D, 4 in 2D) is a template argument, so the candidate's split
+// into (parent, direction) is a division by a constant.
+template <int NDIR>
 __device__ __forceinline__ bool target(const Grow& g, long long c, int& off, int& qi, int& qj, int& qk, int* flag) {
   if (c >= g.ncand) return false;
-  const int m = g.plo + (int)(c / g.ndir), d = (int)(c % g.ndir);
+  const int m = g.plo + (int)(c / NDIR), d = (int)(c % NDIR);
   const int i = g.ix[m - g.base], j = g.iy[m - g.base], k = g.iz ? g.iz[m - g.base] : 0;
   const Box& b = g.nbb;
   if (!b.has(i, j, k)) {
@@ -206,19 +238,23 @@ __device__ __forceinline__ bool target(const Grow& g, long long c, int& off, int
   return true;
 }
 
+template <int NDIR>
 __global__ void __launch_bounds__(kT) claim_kernel(Grow g, int* flag) {
   const long long c = (long long)blockIdx.x * kT + threadIdx.x;
   int off, qi, qj, qk;
-  if (target(g, c, off, qi, qj, qk, flag) && g.nb[off] == 0) atomicMin(&g.claim[off], (int)c);
+  if (target<NDIR>(g, c, off, qi, qj, qk, flag) && g.nb[off] == 0) atomicMin(&g.claim[off], (int)c);
 }
 
+template <int NDIR>
 __global__ void __launch_bounds__(kT) grow_count_kernel(Grow g, int* counts, int* flag) {
   const long long base = (long long)blockIdx.x * kChunk;
   int c = 0;
   for (int it = 0; it < kPer; it++) {
     const long long cand = base + it * kT + threadIdx.x;
     int off, qi, qj, qk;
-    if (target(g, cand, off, qi, qj, qk, flag) && g.claim[off] == (int)cand) c++;
+    const bool w = target<NDIR>(g, cand, off, qi, qj, qk, flag) && g.claim[off] == (int)cand;
+    if (cand < g.ncand) g.win[cand] = w ? 1 : 0;
+    c += w;
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   __shared__ int s[kT / 64];
@@ -233,6 +269,7 @@ __global__ void __launch_bounds__(kT) grow_count_kernel(Grow g, int* counts, int
   }
 }
 
+template <int NDIR>
 __global__ void __launch_bounds__(kT) grow_write_kernel(Grow g, const int* offsets, int* ox, int* oy, int* oz,
                                                         int start_rel, unsigned char mark, int* flag) {
   __shared__ int lds[3][kT / 64];
@@ -241,7 +278,8 @@ __global__ void __launch_bounds__(kT) grow_write_kernel(Grow g, const int* offse
   for (int it = 0; it < kPer; it++) {
     const long long cand = base + it * kT + threadIdx.x;
     int off = 0, qi = 0, qj = 0, qk = 0;
-    const bool win = target(g, cand, off, qi, qj, qk, flag) && g.claim[off] == (int)cand;
+    // the count pass's verdict; only winners find their cell again
+    const bool win = cand < g.ncand && g.win[cand] && target<NDIR>(g, cand, off, qi, qj, qk, flag);
     bool f[3] = {win, false, false};
     int pre[3], tot[3];
     block_prefix3(f, pre, tot, lds);
@@ -264,9 +302,21 @@ __global__ void __launch_bounds__(256) fill_int_kernel(int* p, long long n, int 
 // ----- device scratch of the band builder ---------------------------------------------------------
 int* g_claim = nullptr;
 long long g_claim_n = 0;
+unsigned char* g_win = nullptr;
+long long g_win_n = 0;
 int* g_counts = nullptr;
 long long g_counts_n = 0;
 long long* g_tot = nullptr;
+
+bool ensure_win(const char* who, long long n) {
+  if (n <= g_win_n) return true;
+  if (g_win) (void)hipFree(g_win);
+  g_win = nullptr;
+  g_win_n = 0;
+  if (!check_hip(hipMalloc(&g_win, (size_t)n), who)) return false;
+  g_win_n = n;
+  return true;
+}
 
 bool ensure(const char* who, long long claim_n, long long counts_n) {
   if (claim_n > g_claim_n) {
@@ -315,10 +365,16 @@ bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, in
     const long long ncand = ndir * parents;
     const long long nblk = (ncand + kChunk - 1) / kChunk;
     if (ncand > maxcand) maxcand = ncand;
-    if (!ensure(who, ncells, nblk)) return false;
-    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ndir, ncand, nb, nbb, g_claim};
-    hipLaunchKernelGGL(claim_kernel, dim3((unsigned)((ncand + kT - 1) / kT)), dim3(kT), 0, stream(), g, flag);
-    hipLaunchKernelGGL(grow_count_kernel, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, flag);
+    if (!ensure(who, ncells, nblk) || !ensure_win(who, ncand)) return false;
+    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ndir, ncand, nb, nbb, g_claim, g_win};
+    const dim3 cgrid((unsigned)((ncand + kT - 1) / kT));
+    if (ndir == 6) {
+      hipLaunchKernelGGL(claim_kernel<6>, cgrid, dim3(kT), 0, stream(), g, flag);
+      hipLaunchKernelGGL(grow_count_kernel<6>, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, flag);
+    } else {
+      hipLaunchKernelGGL(claim_kernel<4>, cgrid, dim3(kT), 0, stream(), g, flag);
+      hipLaunchKernelGGL(grow_count_kernel<4>, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, flag);
+    }
     hipLaunchKernelGGL(scan3_kernel, dim3(1), dim3(1024), 0, stream(), g_counts, nblk, g_tot);
     long long tot[3];
     if (!check_hip(hipGetLastError(), who) || !read_totals(who, tot)) return false;
@@ -332,8 +388,12 @@ bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, in
                 (long long)next + tot[0] - 1, nhi);
       return false;
     }
-    hipLaunchKernelGGL(grow_write_kernel, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, ix, iy, iz,
-                       next - nlo, (unsigned char)(L + 1), flag);
+    if (ndir == 6)
+      hipLaunchKernelGGL(grow_write_kernel<6>, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, ix, iy, iz,
+                         next - nlo, (unsigned char)(L + 1), flag);
+    else
+      hipLaunchKernelGGL(grow_write_kernel<4>, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, ix, iy, iz,
+                         next - nlo, (unsigned char)(L + 1), flag);
     if (!check_hip(hipGetLastError(), who)) return false;
     n_lo[L] = next;
     n_hi[L] = next + (int)tot[0] - 1;

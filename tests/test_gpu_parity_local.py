@@ -48,6 +48,33 @@ def test_determine_narrow_band_bitwise(hip, level, seed):
         assert ref.scalars[k] == got.scalars[k], k
 
 
+@pytest.mark.parametrize("level", [1, 3])
+def test_determine_narrow_band_with_a_wider_mark_array(hip, level):
+    """narrow_band on a ghost box wider than phi's (two cells more on some
+    sides, one on others): the marks are addressed through their own box and
+    the cells outside phi's box keep their values; lists and marks bitwise the
+    oracle's."""
+    c = H.band_case(level=level, seed=5)
+    gb = c.boxes["gb"]
+    wide = (gb[0] - 2, gb[1] + 1, gb[2] - 1, gb[3] + 2, gb[4] - 2, gb[5] + 1)
+    c.boxes["nb"] = wide
+    c.arrays["nb"] = np.full(H.shape_of(wide), 9, np.uint8)
+    npts = c.arrays["phi"].size
+    for nm in ("ix", "iy", "iz", "outer"):
+        c.arrays[nm] = np.full(npts, -7, np.int32)
+    c.arrays["n_lo"] = np.full(level + 1, -5, np.int32)
+    c.arrays["n_hi"] = np.full(level + 1, -5, np.int32)
+    c.scalars.update(nlo=0, nhi=npts - 1, level=level)
+    sym = "lsm3ddeterminenarrowband_"
+    a = H.run(H.oracle(), sym, c)
+    b = H.run(hip.api, sym, c, device="cuda")
+    hip.check()
+    for k in ("nb", "ix", "iy", "iz", "outer", "n_lo", "n_hi"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+    assert a[1] == b[1]
+    assert (a[0]["nb"] == 9).sum() > 0
+
+
 def test_band_from_mask_and_outer_check(hip):
     for use_sign in (0, 1):
         c = _local_case(level=3)
