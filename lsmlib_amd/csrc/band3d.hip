@@ -203,8 +203,7 @@ struct Grow {
   const int *ix, *iy, *iz;  // element 0 is index base; iz null for a 2D band
   int base;
   int plo;                  // first parent index
-  int ndir;                 // neighbour directions: 6 in 3D, 4 in 2D
-  long long ncand;          // ndir * parents
+  long long ncand;          // directions (6 in 3D, 4 in 2D) x parents
   unsigned char* nb;
   Box nbb;
   int* claim;               // over nbb
@@ -366,7 +365,7 @@ bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, in
     const long long nblk = (ncand + kChunk - 1) / kChunk;
     if (ncand > maxcand) maxcand = ncand;
     if (!ensure(who, ncells, nblk) || !ensure_win(who, ncand)) return false;
-    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ndir, ncand, nb, nbb, g_claim, g_win};
+    Grow g{ix, iy, iz, nlo, n_lo[L - 1], ncand, nb, nbb, g_claim, g_win};
     const dim3 cgrid((unsigned)((ncand + kT - 1) / kT));
     if (ndir == 6) {
       hipLaunchKernelGGL(claim_kernel<6>, cgrid, dim3(kT), 0, stream(), g, flag);
