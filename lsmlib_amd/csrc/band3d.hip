@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kPer = 16;               // items per thread in a chunk
+static_assert(3 * kPer <= 64, "level-0 flags of a thread are kept in one 64-bit mask");
 constexpr int kChunk = kT * kPer;      // items per block
 constexpr int kNoClaim = 0x7fffffff;
 
@@ -34,27 +35,6 @@ __device__ __forceinline__ int wave_prefix(bool f, int* total) {
   const unsigned long long m = __ballot(f);
   *total = __popcll(m);
   return __popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull));
-}
-
-// Block-wide exclusive prefix of up to three flags for one pass of 256
-// threads; returns per-flag prefixes and writes block totals.
-__device__ void block_prefix3(const bool f[3], int pre[3], int tot[3], int (*lds)[kT / 64]) {
-  const int w = threadIdx.x >> 6;
-  int wt[3], wp[3];
-  for (int q = 0; q < 3; q++) wp[q] = wave_prefix(f[q], &wt[q]);
-  if ((threadIdx.x & 63) == 0)
-    for (int q = 0; q < 3; q++) lds[q][w] = wt[q];
-  __syncthreads();
-  for (int q = 0; q < 3; q++) {
-    int before = 0, all = 0;
-    for (int v = 0; v < kT / 64; v++) {
-      if (v < w) before += lds[q][v];
-      all += lds[q][v];
-    }
-    pre[q] = before + wp[q];
-    tot[q] = all;
-  }
-  __syncthreads();
 }
 
 // ----- level 0 ------------------------------------------------------------------------------------
@@ -66,28 +46,74 @@ struct Level0 {
   Box B;
   real_t width, width_inner;
   int mode;           // 0: |phi| < width; 1: mask >= 0; 2: mask <= 0; 3: mask < 0
-  __device__ void flags(long long t, bool f[3]) const {
+  // the value that decides cell t (phi, or the mask), 0 past the end
+  __device__ real_t load(long long t) const {
+    if (t >= n) return real_t(0);
+    return mode == 0 ? phi[t] : mask[t];
+  }
+  __device__ void flags_of(long long t, real_t v, bool f[3]) const {
     f[0] = f[1] = f[2] = false;
     if (t >= n) return;
     if (mode == 0) {
-      const real_t v = phi[t], a = fabs(v);
+      const real_t a = fabs(v);
       f[0] = a < width;
       const bool outer = f[0] && a >= width_inner;
       f[1] = outer && v <= real_t(0);
       f[2] = outer && !(v <= real_t(0));
     } else {
-      const real_t v = mask[t];
       f[0] = mode == 1 ? v >= real_t(0) : (mode == 2 ? v <= real_t(0) : v < real_t(0));
     }
   }
 };
 
+// Per-block placement of up to three flags over kPer passes of kT items
+// (item of pass it and thread x: base + it * kT + x).  Pass 1 loads and flags
+// every item, and each wave's count per pass and flag goes to LDS; after one
+// barrier, pass 2 places the items of each pass in item order: the block
+// offset is the counts of the earlier passes and of the earlier waves of the
+// same pass, plus the lane's prefix within its wave (a ballot).
+struct PassCounts {
+  int c[kPer][kT / 64][3];
+};
+__device__ __forceinline__ void record_pass(PassCounts& pc, int it, const bool f[3]) {
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const int tot = __popcll(__ballot(f[q]));
+    if ((threadIdx.x & 63) == 0) pc.c[it][w][q] = tot;
+  }
+}
+// Block offsets of this thread's items of pass it (flags f) given the totals
+// of the passes before it (run, advanced past pass it on return).  Every lane
+// of the wave must call it (it ballots).
+__device__ __forceinline__ void place_pass(const PassCounts& pc, int it, const bool f[3], int run[3], int off[3]) {
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    int tot;
+    const int lane_pre = wave_prefix(f[q], &tot);
+    int before = 0, all = 0;
+#pragma unroll
+    for (int v = 0; v < kT / 64; v++) {
+      const int x = pc.c[it][v][q];
+      before += v < w ? x : 0;
+      all += x;
+    }
+    off[q] = run[q] + before + lane_pre;
+    run[q] += all;
+  }
+}
+
 __global__ void __launch_bounds__(kT) level0_count_kernel(Level0 a, int* counts) {
   const long long base = (long long)blockIdx.x * kChunk;
   int c[3] = {0, 0, 0};
+  real_t v[kPer];  // every load of the chunk issued before any flag is formed
+#pragma unroll
+  for (int it = 0; it < kPer; it++) v[it] = a.load(base + it * kT + threadIdx.x);
+#pragma unroll
   for (int it = 0; it < kPer; it++) {
     bool f[3];
-    a.flags(base + it * kT + threadIdx.x, f);
+    a.flags_of(base + it * kT + threadIdx.x, v[it], f);
     for (int q = 0; q < 3; q++) c[q] += f[q];
   }
   for (int q = 0; q < 3; q++)
@@ -155,12 +181,26 @@ struct Level0Out {
 };
 
 __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* offsets, Level0Out o) {
-  __shared__ int lds[3][kT / 64];
+  __shared__ PassCounts pc;
   const long long base = (long long)blockIdx.x * kChunk;
+  const long long t0 = base + threadIdx.x;
+  unsigned long long fl = 0;  // bit 3 it + q: flag q of pass it (48 bits)
+  {
+    real_t v[kPer];  // every load of the chunk issued before any flag is formed
+#pragma unroll
+    for (int it = 0; it < kPer; it++) v[it] = a.load(t0 + it * kT);
+#pragma unroll
+    for (int it = 0; it < kPer; it++) {
+      bool f[3];
+      a.flags_of(t0 + it * kT, v[it], f);
+      record_pass(pc, it, f);
+      for (int q = 0; q < 3; q++) fl |= (unsigned long long)f[q] << (3 * it + q);
+    }
+  }
+  __syncthreads();
   int run[3] = {offsets[3 * blockIdx.x], offsets[3 * blockIdx.x + 1], offsets[3 * blockIdx.x + 2]};
   // (i, j, k) of the thread's first cell, then advanced by kT cells per pass:
   // one division per thread instead of three per cell
-  const long long t0 = base + threadIdx.x;
   int i = a.B.ilo + (int)(t0 % a.nx);
   int j = a.B.jlo + (int)((t0 / a.nx) % a.ny);
   int k = a.B.klo + (int)(t0 / ((long long)a.nx * a.ny));
@@ -168,25 +208,24 @@ __global__ void __launch_bounds__(kT) level0_write_kernel(Level0 a, const int* o
                     o.nbb.klo == a.B.klo && o.nbb.khi == a.B.khi;
   for (int it = 0; it < kPer; it++) {
     const long long t = t0 + it * kT;
-    bool f[3];
-    a.flags(t, f);
-    int pre[3], tot[3];
-    block_prefix3(f, pre, tot, lds);
+    const bool f[3] = {((fl >> (3 * it)) & 1ull) != 0, ((fl >> (3 * it + 1)) & 1ull) != 0,
+                       ((fl >> (3 * it + 2)) & 1ull) != 0};
+    int off[3];
+    place_pass(pc, it, f, run, off);
     if (t < a.n) {
       const long long nbo = same ? t
                                  : (long long)(i - o.nbb.ilo) +
                                        (long long)o.nbb.nx() * ((j - o.nbb.jlo) + (long long)o.nbb.ny() * (k - o.nbb.klo));
       o.nb[nbo] = f[0] ? 1 : 0;
       if (f[0]) {
-        const int r = run[0] + pre[0];
+        const int r = off[0];
         o.ix[r] = i;
         o.iy[r] = j;
         if (o.iz) o.iz[r] = k;
-        if (f[1]) o.outer[run[1] + pre[1]] = o.nlo + r;
-        if (f[2]) o.outer[(o.nhi_outer - o.nlo_outer) - (run[2] + pre[2])] = o.nlo + r;
+        if (f[1]) o.outer[off[1]] = o.nlo + r;
+        if (f[2]) o.outer[(o.nhi_outer - o.nlo_outer) - off[2]] = o.nlo + r;
       }
     }
-    for (int q = 0; q < 3; q++) run[q] += tot[q];
     i += kT;
     while (i > a.B.ihi) {
       i -= a.nx;
@@ -207,7 +246,7 @@ struct Grow {
   unsigned char* nb;
   Box nbb;
   int* claim;               // over nbb
-  unsigned char* win;       // per candidate: 1 where it won its cell (written by the count pass)
+  unsigned char* win;       // per candidate: tried to claim a free cell (claim pass), then won it (count pass)
 };
 
 // NDIR (6 in 3D, 4 in 2D) is a template argument, so the candidate's split
@@ -241,7 +280,12 @@ template <int NDIR>
 __global__ void __launch_bounds__(kT) claim_kernel(Grow g, int* flag) {
   const long long c = (long long)blockIdx.x * kT + threadIdx.x;
   int off, qi, qj, qk;
-  if (target<NDIR>(g, c, off, qi, qj, qk, flag) && g.nb[off] == 0) atomicMin(&g.claim[off], (int)c);
+  bool tried = false;
+  if (target<NDIR>(g, c, off, qi, qj, qk, flag) && g.nb[off] == 0) {
+    atomicMin(&g.claim[off], (int)c);
+    tried = true;
+  }
+  if (c < g.ncand) g.win[c] = tried ? 1 : 0;  // the count pass looks only at these
 }
 
 template <int NDIR>
@@ -250,9 +294,13 @@ __global__ void __launch_bounds__(kT) grow_count_kernel(Grow g, int* counts, int
   int c = 0;
   for (int it = 0; it < kPer; it++) {
     const long long cand = base + it * kT + threadIdx.x;
-    int off, qi, qj, qk;
-    const bool w = target<NDIR>(g, cand, off, qi, qj, qk, flag) && g.claim[off] == (int)cand;
-    if (cand < g.ncand) g.win[cand] = w ? 1 : 0;
+    // only candidates that tried to claim a free cell can have won it
+    bool w = false;
+    if (cand < g.ncand && g.win[cand]) {
+      int off, qi, qj, qk;
+      w = target<NDIR>(g, cand, off, qi, qj, qk, flag) && g.claim[off] == (int)cand;
+      g.win[cand] = w ? 1 : 0;
+    }
     c += w;
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -271,26 +319,37 @@ __global__ void __launch_bounds__(kT) grow_count_kernel(Grow g, int* counts, int
 template <int NDIR>
 __global__ void __launch_bounds__(kT) grow_write_kernel(Grow g, const int* offsets, int* ox, int* oy, int* oz,
                                                         int start_rel, unsigned char mark, int* flag) {
-  __shared__ int lds[3][kT / 64];
+  __shared__ PassCounts pc;
   const long long base = (long long)blockIdx.x * kChunk;
-  int run = offsets[3 * blockIdx.x];
-  for (int it = 0; it < kPer; it++) {
-    const long long cand = base + it * kT + threadIdx.x;
-    int off = 0, qi = 0, qj = 0, qk = 0;
-    // the count pass's verdict; only winners find their cell again
-    const bool win = cand < g.ncand && g.win[cand] && target<NDIR>(g, cand, off, qi, qj, qk, flag);
-    bool f[3] = {win, false, false};
-    int pre[3], tot[3];
-    block_prefix3(f, pre, tot, lds);
-    if (win) {
-      const int r = start_rel + run + pre[0];
-      ox[r] = qi;
-      oy[r] = qj;
-      if (oz) oz[r] = qk;
-      g.nb[off] = mark;
-      g.claim[off] = kNoClaim;
+  const long long c0 = base + threadIdx.x;
+  unsigned wins = 0;  // bit it: the count pass's verdict for pass it, all loaded first
+  {
+    unsigned char wb[kPer];
+#pragma unroll
+    for (int it = 0; it < kPer; it++) wb[it] = c0 + it * kT < g.ncand ? g.win[c0 + it * kT] : 0;
+#pragma unroll
+    for (int it = 0; it < kPer; it++) {
+      const bool f[3] = {wb[it] != 0, false, false};
+      record_pass(pc, it, f);
+      wins |= (unsigned)(wb[it] != 0) << it;
     }
-    run += tot[0];
+  }
+  __syncthreads();
+  int run[3] = {offsets[3 * blockIdx.x], 0, 0};
+  for (int it = 0; it < kPer; it++) {
+    const bool f[3] = {((wins >> it) & 1u) != 0, false, false};
+    int off[3];
+    place_pass(pc, it, f, run, off);
+    if (!f[0]) continue;
+    const long long cand = c0 + it * kT;
+    int off_cell = 0, qi = 0, qj = 0, qk = 0;
+    if (!target<NDIR>(g, cand, off_cell, qi, qj, qk, flag)) continue;  // cannot happen for a winner
+    const int r = start_rel + off[0];
+    ox[r] = qi;
+    oy[r] = qj;
+    if (oz) oz[r] = qk;
+    g.nb[off_cell] = mark;
+    g.claim[off_cell] = kNoClaim;
   }
 }
 
