@@ -276,16 +276,36 @@ __device__ __forceinline__ bool target(const Grow& g, long long c, int& off, int
   return true;
 }
 
+// One thread per parent: its index read once, the marks of its NDIR
+// neighbours loaded together, then a claim for each free one (candidate
+// NDIR * parent + direction, as target() numbers them).
 template <int NDIR>
 __global__ void __launch_bounds__(kT) claim_kernel(Grow g, int* flag) {
-  const long long c = (long long)blockIdx.x * kT + threadIdx.x;
-  int off, qi, qj, qk;
-  bool tried = false;
-  if (target<NDIR>(g, c, off, qi, qj, qk, flag) && g.nb[off] == 0) {
-    atomicMin(&g.claim[off], (int)c);
-    tried = true;
+  const long long p = (long long)blockIdx.x * kT + threadIdx.x;
+  if (p >= g.ncand / NDIR) return;
+  const long long c0 = NDIR * p;
+  const int m = g.plo + (int)p;
+  const int i = g.ix[m - g.base], j = g.iy[m - g.base], k = g.iz ? g.iz[m - g.base] : 0;
+  const Box& b = g.nbb;
+  if (!b.has(i, j, k)) {
+    *flag = 1;
+#pragma unroll
+    for (int d = 0; d < NDIR; d++) g.win[c0 + d] = 0;
+    return;
   }
-  if (c < g.ncand) g.win[c] = tried ? 1 : 0;  // the count pass looks only at these
+  const bool ok[6] = {i < b.ihi, i > b.ilo, j < b.jhi, j > b.jlo, k < b.khi, k > b.klo};
+  const int sy = b.nx(), sz = b.nx() * b.ny();
+  const int o = (i - b.ilo) + sy * ((j - b.jlo) + b.ny() * (k - b.klo));
+  const int step[6] = {1, -1, sy, -sy, sz, -sz};
+  unsigned char mk[NDIR];
+#pragma unroll
+  for (int d = 0; d < NDIR; d++) mk[d] = ok[d] ? g.nb[o + step[d]] : 1;
+#pragma unroll
+  for (int d = 0; d < NDIR; d++) {
+    const bool tried = ok[d] && mk[d] == 0;
+    if (tried) atomicMin(&g.claim[o + step[d]], (int)(c0 + d));
+    g.win[c0 + d] = tried ? 1 : 0;  // the count pass looks only at these
+  }
 }
 
 template <int NDIR>
@@ -425,7 +445,7 @@ bool grow_levels(const char* who, unsigned char* nb, const Box& nbb, int* ix, in
     if (ncand > maxcand) maxcand = ncand;
     if (!ensure(who, ncells, nblk) || !ensure_win(who, ncand)) return false;
     Grow g{ix, iy, iz, nlo, n_lo[L - 1], ncand, nb, nbb, g_claim, g_win};
-    const dim3 cgrid((unsigned)((ncand + kT - 1) / kT));
+    const dim3 cgrid((unsigned)((parents + kT - 1) / kT));  // one thread per parent
     if (ndir == 6) {
       hipLaunchKernelGGL(claim_kernel<6>, cgrid, dim3(kT), 0, stream(), g, flag);
       hipLaunchKernelGGL(grow_count_kernel<6>, dim3((unsigned)nblk), dim3(kT), 0, stream(), g, g_counts, flag);
