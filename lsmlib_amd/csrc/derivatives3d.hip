@@ -37,6 +37,8 @@ __global__ void __launch_bounds__(256) hj_plus_minus_kernel(Box fb, Axes3 plus, 
   const real_t inv[3] = {ix, iy, iz};
   // All three lines are loaded before the first store: an output may alias phi
   // as far as the compiler knows, so a load after a store would wait for it.
+  // (One cell per thread: a pair of cells along k measured slower here, the
+  // registers of two full crosses costing more occupancy than they gain.)
   real_t f[3][7];
 #pragma unroll
   for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f[d]);
@@ -53,28 +55,55 @@ template <int ORDER>
 __global__ void __launch_bounds__(256) hj_upwind_kernel(Box fb, Axes3 out, View<const real_t> phi,
                                                         View<const real_t> vx, View<const real_t> vy,
                                                         View<const real_t> vz, real_t ix, real_t iy, real_t iz) {
-  LSMHIP_CELL(fb);
+  LSMHIP_CELL_PAIR(fb);
   const real_t inv[3] = {ix, iy, iz};
-  const real_t vel[3] = {vx(i, j, k), vy(i, j, k), vz(i, j, k)};
-  // Every line is loaded (inside phi's checked box) before the first store,
-  // also where the speed below zero_tol leaves it unused.
-  real_t f[3][7];
+  // Every line of both cells is loaded (inside phi's checked box) before the
+  // first store, also where the speed below zero_tol leaves it unused.
+  real_t vel[2][3], f[2][3][7];
 #pragma unroll
-  for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f[d]);
+  for (int c = 0; c < 2; c++)
+    if (c < nk) {
+      vel[c][0] = vx(i, j, k0 + c);
+      vel[c][1] = vy(i, j, k0 + c);
+      vel[c][2] = vz(i, j, k0 + c);
 #pragma unroll
-  for (int d = 0; d < 3; d++)
-    out.o[d](i, j, k) = fabs(vel[d]) < kZeroTol ? real_t(0) : stencil::upwind<ORDER>(f[d], inv[d], vel[d] > real_t(0));
+      for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k0 + c, d, f[c][d]);
+    }
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+    if (c < nk)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        out.o[d](i, j, k0 + c) = fabs(vel[c][d]) < kZeroTol
+                                     ? real_t(0)
+                                     : stencil::upwind<ORDER>(f[c][d], inv[d], vel[c][d] > real_t(0));
 }
 
 __global__ void __launch_bounds__(256) central2_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
                                                        real_t fy, real_t fz) {
-  LSMHIP_CELL(fb);
-  // loads first: a load after a store to a possibly aliasing output would wait for it
-  const real_t xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, j + 1, k), ym = phi(i, j - 1, k),
-               zp = phi(i, j, k + 1), zm = phi(i, j, k - 1);
-  out.o[0](i, j, k) = (xp - xm) * fx;
-  out.o[1](i, j, k) = (yp - ym) * fy;
-  out.o[2](i, j, k) = (zp - zm) * fz;
+  LSMHIP_CELL_PAIR(fb);
+  // loads of both cells first: a load after a store to a possibly aliasing
+  // output would wait for it
+  real_t xp[2], xm[2], yp[2], ym[2], zp[2], zm[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+    if (c < nk) {
+      const int k = k0 + c;
+      xp[c] = phi(i + 1, j, k);
+      xm[c] = phi(i - 1, j, k);
+      yp[c] = phi(i, j + 1, k);
+      ym[c] = phi(i, j - 1, k);
+      zp[c] = phi(i, j, k + 1);
+      zm[c] = phi(i, j, k - 1);
+    }
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+    if (c < nk) {
+      const int k = k0 + c;
+      out.o[0](i, j, k) = (xp[c] - xm[c]) * fx;
+      out.o[1](i, j, k) = (yp[c] - ym[c]) * fy;
+      out.o[2](i, j, k) = (zp[c] - zm[c]) * fz;
+    }
 }
 
 __global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
@@ -91,11 +120,26 @@ __global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<c
 
 __global__ void __launch_bounds__(256) laplacian2_kernel(Box fb, View<real_t> out, View<const real_t> phi,
                                                          real_t ax, real_t ay, real_t az) {
-  LSMHIP_CELL(fb);
-  const real_t c = phi(i, j, k);
-  out(i, j, k) = ax * (phi(i + 1, j, k) - real_t(2) * c + phi(i - 1, j, k)) +
-                 ay * (phi(i, j + 1, k) - real_t(2) * c + phi(i, j - 1, k)) +
-                 az * (phi(i, j, k + 1) - real_t(2) * c + phi(i, j, k - 1));
+  LSMHIP_CELL_PAIR(fb);
+  real_t v[2][7];  // centre, x+, x-, y+, y-, z+, z- of both cells, loaded first
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+    if (c < nk) {
+      const int k = k0 + c;
+      v[c][0] = phi(i, j, k);
+      v[c][1] = phi(i + 1, j, k);
+      v[c][2] = phi(i - 1, j, k);
+      v[c][3] = phi(i, j + 1, k);
+      v[c][4] = phi(i, j - 1, k);
+      v[c][5] = phi(i, j, k + 1);
+      v[c][6] = phi(i, j, k - 1);
+    }
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+    if (c < nk)
+      out(i, j, k0 + c) = ax * (v[c][1] - real_t(2) * v[c][0] + v[c][2]) +
+                          ay * (v[c][3] - real_t(2) * v[c][0] + v[c][4]) +
+                          az * (v[c][5] - real_t(2) * v[c][0] + v[c][6]);
 }
 
 __global__ void __launch_bounds__(256) average_grad_kernel(Box fb, Axes3 out, Axes3 plus, Axes3 minus) {
@@ -143,7 +187,7 @@ void hj_upwind(const char* who, real_t* px, real_t* py, real_t* pz, const Box& g
       !check_box(pg, fb.widen(w, w, w), who, "phi") || !check_size(pg, who) || !check_size(g, who) ||
       !check_size(vg, who))
     return;
-  hipLaunchKernelGGL(hj_upwind_kernel<ORDER>, grid_for(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g),
+  hipLaunchKernelGGL(hj_upwind_kernel<ORDER>, grid_for_pair(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g),
                      view(phi, pg), view(vx, vg), view(vy, vg), view(vz, vg), real_t(1.0) / dx, real_t(1.0) / dy,
                      real_t(1.0) / dz);
   finish(who);
@@ -209,7 +253,7 @@ LSMHIP_EXPORT void lsm3dcentralgradorder2_(real_t* px, real_t* py, real_t* pz, L
   if (!check_box(g, fb, who, "grad_phi") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") || !check_size(g, who) ||
       !check_size(pg, who))
     return;
-  hipLaunchKernelGGL(central2_kernel, grid_for(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g), view(phi, pg),
+  hipLaunchKernelGGL(central2_kernel, grid_for_pair(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g), view(phi, pg),
                      real_t(0.5) / *dx, real_t(0.5) / *dy, real_t(0.5) / *dz);
   finish(who);
 }
@@ -237,7 +281,7 @@ LSMHIP_EXPORT void lsm3dlaplacianorder2_(real_t* lap, LSMHIP_BOXP(lg), const rea
   if (!check_box(lg, fb, who, "laplacian_phi") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") ||
       !check_size(lg, who) || !check_size(pg, who))
     return;
-  hipLaunchKernelGGL(laplacian2_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(lap, lg), view(phi, pg),
+  hipLaunchKernelGGL(laplacian2_kernel, grid_for_pair(fb), block2d(), 0, stream(), fb, view(lap, lg), view(phi, pg),
                      real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz);
   finish(who);
 }

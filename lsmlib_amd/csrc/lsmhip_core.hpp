@@ -127,6 +127,21 @@ inline dim3 grid_for(const Box& b) {
 }
 inline dim3 block2d() { return dim3(kBX, kBY, 1); }
 
+// Two cells along k per thread, (i, j, k0) and (i, j, k0 + 1) (nk of them inside
+// the box), on the same tiles and XCD slabs as LSMHIP_CELL: for stencil
+// kernels that issue the loads of both cells before either store, so each
+// wave keeps twice the bytes in flight.
+inline dim3 grid_for_pair(const Box& b) {
+  const dim3 g = grid_for(b);
+  return dim3(g.x, g.y, (unsigned)((b.nz() + 1) / 2));
+}
+#define LSMHIP_CELL_PAIR(b)                                                                       \
+  const int i = (b).ilo + (int)(blockIdx.x / kXcds) * kBX + (int)threadIdx.x;                    \
+  const int j = (b).jlo + (int)((blockIdx.x % kXcds) * gridDim.y + blockIdx.y) * kBY + (int)threadIdx.y; \
+  const int k0 = (b).klo + 2 * (int)blockIdx.z;                                                   \
+  const int nk = (b).khi - k0 + 1 < 2 ? (b).khi - k0 + 1 : 2;                                     \
+  if (i > (b).ihi || j > (b).jhi) return
+
 #define LSMHIP_CELL(b)                                                                            \
   const int i = (b).ilo + (int)(blockIdx.x / kXcds) * kBX + (int)threadIdx.x;                    \
   const int j = (b).jlo + (int)((blockIdx.x % kXcds) * gridDim.y + blockIdx.y) * kBY + (int)threadIdx.y; \
