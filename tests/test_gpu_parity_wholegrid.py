@@ -83,6 +83,15 @@ def test_zero_out_whole_ghost_box_at_any_alignment(prec, offset, gb):
         assert np.all(h[:offset] == 7.5) and np.all(h[offset + vol:] == 7.5)
 
 
+@pytest.mark.parametrize("sym", WHOLE_GRID)
+def test_whole_grid_bitwise_odd_plane_count(hip, sym):
+    """A fill box of 11 planes along k (and 19 x 13 in-plane): kernels that
+    take two cells along k per thread end on a pair holding one cell."""
+    c = H.grid_case(4, gb=(-3, 21, -3, 15, -3, 13), fb=(0, 18, 0, 12, 0, 10))
+    _prepare(c, sym)
+    _cmp(hip, sym, c)
+
+
 @pytest.mark.parametrize("sym", QUIRKY_BOUNDS)
 def test_quirky_loop_bounds(hip, sym):
     c = H.grid_case(3, mixed_boxes=False)
