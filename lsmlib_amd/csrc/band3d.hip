@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(kT) grow_write_kernel(Grow g, const int* offse
   __shared__ PassCounts pc;
   const long long base = (long long)blockIdx.x * kChunk;
   const long long c0 = base + threadIdx.x;
-  unsigned wins = 0;  // bit it: the count pass's verdict for pass it, all loaded first
+  unsigned wins = 0;  // bit it: the count pass's result for pass it, all loaded first
   {
     unsigned char wb[kPer];
 #pragma unroll
