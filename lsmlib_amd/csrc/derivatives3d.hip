@@ -155,8 +155,13 @@ __global__ void __launch_bounds__(256) average_grad_kernel(Box fb, Axes3 out, Ax
 }
 
 __global__ void __launch_bounds__(256) grad_mag_kernel(Box fb, Axes3 g, View<real_t> mag) {
-  LSMHIP_CELL(fb);
-  mag(i, j, k) = stencil::grad_magnitude(g.o[0](i, j, k), g.o[1](i, j, k), g.o[2](i, j, k), real_t(1.e-8));
+  LSMHIP_CELLS(fb, kPointNK);
+  struct V {
+    real_t x, y, z;
+  };
+  pair_cells(
+      k0, nk, [&](int k) { return V{g.o[0](i, j, k), g.o[1](i, j, k), g.o[2](i, j, k)}; },
+      [&](int k, const V& a) { mag(i, j, k) = stencil::grad_magnitude(a.x, a.y, a.z, real_t(1.e-8)); });
 }
 
 Axes3 axes(real_t* x, real_t* y, real_t* z, const Box& b) { return Axes3{{view(x, b), view(y, b), view(z, b)}}; }
@@ -307,7 +312,7 @@ LSMHIP_EXPORT void lsm3dgradientmagnitude_(const real_t* px, const real_t* py, c
   Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g);
   if (fb.empty()) return;
   if (!check_box(g, fb, who, "grad_phi") || !check_size(g, who)) return;
-  hipLaunchKernelGGL(grad_mag_kernel, grid_for(fb), block2d(), 0, stream(), fb,
+  hipLaunchKernelGGL(grad_mag_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb,
                      axes((real_t*)px, (real_t*)py, (real_t*)pz, g), view(mag, g));
   finish(who);
 }

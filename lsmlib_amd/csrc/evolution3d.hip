@@ -22,20 +22,35 @@ __global__ void __launch_bounds__(256) advection_kernel(Box fb, View<real_t> rhs
                                                         View<const real_t> py, View<const real_t> pz,
                                                         View<const real_t> vx, View<const real_t> vy,
                                                         View<const real_t> vz) {
-  LSMHIP_CELL(fb);
-  rhs(i, j, k) = rhs(i, j, k) - (vx(i, j, k) * px(i, j, k) + vy(i, j, k) * py(i, j, k) + vz(i, j, k) * pz(i, j, k));
+  LSMHIP_CELLS(fb, kPointNK);
+  struct V {
+    real_t r, u, x, v, y, w, z;
+  };
+  pair_cells(
+      k0, nk,
+      [&](int k) { return V{rhs(i, j, k), vx(i, j, k), px(i, j, k), vy(i, j, k), py(i, j, k), vz(i, j, k), pz(i, j, k)}; },
+      [&](int k, const V& a) { rhs(i, j, k) = a.r - (a.u * a.x + a.v * a.y + a.w * a.z); });
 }
 
 // vconst is used when vn.p is null (constant normal speed).
 __global__ void __launch_bounds__(256) normal_kernel(Box fb, View<real_t> rhs, Six g, View<const real_t> vn,
                                                      real_t vconst) {
-  LSMHIP_CELL(fb);
-  const real_t v = vn.p ? vn(i, j, k) : vconst;
-  if (fabs(v) >= kZeroTol) {
-    const real_t s = stencil::godunov_sq(v > real_t(0), g.xp(i, j, k), g.yp(i, j, k), g.zp(i, j, k), g.xm(i, j, k),
-                                         g.ym(i, j, k), g.zm(i, j, k));
-    rhs(i, j, k) = rhs(i, j, k) - v * sqrt(s);
-  }
+  LSMHIP_CELLS(fb, kPointNK);
+  // every input of both cells is loaded first (also where the speed is below
+  // zero_tol and the cell is left as it is)
+  struct V {
+    real_t v, xp, yp, zp, xm, ym, zm, r;
+  };
+  pair_cells(
+      k0, nk,
+      [&](int k) {
+        return V{vn.p ? vn(i, j, k) : vconst, g.xp(i, j, k), g.yp(i, j, k), g.zp(i, j, k), g.xm(i, j, k),
+                 g.ym(i, j, k), g.zm(i, j, k), rhs(i, j, k)};
+      },
+      [&](int k, const V& a) {
+        if (fabs(a.v) >= kZeroTol)
+          rhs(i, j, k) = a.r - a.v * sqrt(stencil::godunov_sq(a.v > real_t(0), a.xp, a.yp, a.zp, a.xm, a.ym, a.zm));
+      });
 }
 
 __global__ void __launch_bounds__(256) const_curv_kernel(Box fb, View<real_t> rhs, View<const real_t> px,
@@ -79,10 +94,19 @@ __global__ void __launch_bounds__(256) ext_normal_kernel(Box lb, View<real_t> rh
 
 __global__ void __launch_bounds__(256) reinit_kernel(Box fb, View<real_t> out, View<const real_t> phi,
                                                      View<const real_t> phi0, Six g, real_t dx_sq, int use_phi0) {
-  LSMHIP_CELL(fb);
-  const real_t s = use_phi0 ? phi0(i, j, k) : phi(i, j, k);
-  out(i, j, k) = stencil::reinit_rhs(s, use_phi0 != 0, g.xp(i, j, k), g.yp(i, j, k), g.zp(i, j, k), g.xm(i, j, k),
-                                     g.ym(i, j, k), g.zm(i, j, k), dx_sq);
+  LSMHIP_CELLS(fb, kPointNK);
+  struct V {
+    real_t s, xp, yp, zp, xm, ym, zm;
+  };
+  pair_cells(
+      k0, nk,
+      [&](int k) {
+        return V{use_phi0 ? phi0(i, j, k) : phi(i, j, k), g.xp(i, j, k), g.yp(i, j, k), g.zp(i, j, k), g.xm(i, j, k),
+                 g.ym(i, j, k), g.zm(i, j, k)};
+      },
+      [&](int k, const V& a) {
+        out(i, j, k) = stencil::reinit_rhs(a.s, use_phi0 != 0, a.xp, a.yp, a.zp, a.xm, a.ym, a.zm, dx_sq);
+      });
 }
 
 struct CurvFactors {
@@ -116,13 +140,18 @@ __global__ void __launch_bounds__(256) mean_curv_kernel(Box fb, View<real_t> kap
 template <int KIND>
 __global__ void __launch_bounds__(256) rk_kernel(Box fb, View<real_t> out, View<const real_t> us,
                                                  View<const real_t> uc, View<const real_t> rhs, real_t dt) {
-  LSMHIP_CELL(fb);
-  const real_t r = rhs(i, j, k);
-  if (KIND == 1) out(i, j, k) = uc(i, j, k) + dt * r;
-  if (KIND == 2) out(i, j, k) = real_t(0.5) * (uc(i, j, k) + us(i, j, k) + dt * r);
-  if (KIND == 32) out(i, j, k) = real_t(0.75) * uc(i, j, k) + real_t(0.25) * (us(i, j, k) + dt * r);
-  if (KIND == 33)
-    out(i, j, k) = real_t(1.0 / 3.0) * uc(i, j, k) + real_t(2.0 / 3.0) * (us(i, j, k) + dt * r);
+  LSMHIP_CELLS(fb, kPointNK);
+  struct V {
+    real_t r, c, s;  // rhs, u_cur, u_stage (not read for RK1)
+  };
+  pair_cells(
+      k0, nk, [&](int k) { return V{rhs(i, j, k), uc(i, j, k), KIND == 1 ? real_t(0) : us(i, j, k)}; },
+      [&](int k, const V& a) {
+        if (KIND == 1) out(i, j, k) = a.c + dt * a.r;
+        if (KIND == 2) out(i, j, k) = real_t(0.5) * (a.c + a.s + dt * a.r);
+        if (KIND == 32) out(i, j, k) = real_t(0.75) * a.c + real_t(0.25) * (a.s + dt * a.r);
+        if (KIND == 33) out(i, j, k) = real_t(1.0 / 3.0) * a.c + real_t(2.0 / 3.0) * (a.s + dt * a.r);
+      });
 }
 
 bool boxes_ok(const char* who, const Box& fb, std::initializer_list<std::pair<Box, const char*>> arrays) {
@@ -137,7 +166,7 @@ void rk_stage(const char* who, real_t* out, const Box& og, const real_t* us, con
   if (fb.empty()) return;
   if (!boxes_ok(who, fb, {{og, "u_next"}, {cg, "u_cur"}, {rg, "rhs"}})) return;
   if (KIND != 1 && !boxes_ok(who, fb, {{sg, "u_stage"}})) return;
-  hipLaunchKernelGGL(rk_kernel<KIND>, grid_for(fb), block2d(), 0, stream(), fb, view(out, og), view(us, sg),
+  hipLaunchKernelGGL(rk_kernel<KIND>, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(out, og), view(us, sg),
                      view(uc, cg), view(rhs, rg), dt);
   finish(who);
 }
@@ -162,7 +191,7 @@ LSMHIP_EXPORT void lsm3daddadvectiontermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), 
   const char* who = "lsm3daddadvectiontermtolserhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), g = LSMHIP_BOX(g), vg = LSMHIP_BOX(vg);
   if (fb.empty() || !boxes_ok(who, fb, {{rg, "lse_rhs"}, {g, "grad_phi"}, {vg, "vel"}})) return;
-  hipLaunchKernelGGL(advection_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(rhs, rg), view(px, g),
+  hipLaunchKernelGGL(advection_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(rhs, rg), view(px, g),
                      view(py, g), view(pz, g), view(vx, vg), view(vy, vg), view(vz, vg));
   finish(who);
 }
@@ -177,7 +206,7 @@ LSMHIP_EXPORT void lsm3daddnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), 
   if (fb.empty() ||
       !boxes_ok(who, fb, {{rg, "lse_rhs"}, {gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}, {vg, "vel_n"}}))
     return;
-  hipLaunchKernelGGL(normal_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(rhs, rg),
+  hipLaunchKernelGGL(normal_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(rhs, rg),
                      six(pxp, pyp, pzp, gp, pxm, pym, pzm, gm), view(vn, vg), real_t(0));
   finish(who);
 }
@@ -191,7 +220,7 @@ LSMHIP_EXPORT void lsm3daddconstnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP(
   if (fb.empty() || !(fabs(*vel_n) >= kZeroTol)) return;
   if (!boxes_ok(who, fb, {{rg, "lse_rhs"}, {gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}})) return;
   View<const real_t> none{nullptr, 0, 0, 0, 0, 0};
-  hipLaunchKernelGGL(normal_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(rhs, rg),
+  hipLaunchKernelGGL(normal_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(rhs, rg),
                      six(pxp, pyp, pzp, gp, pxm, pym, pzm, gm), none, *vel_n);
   finish(who);
 }
@@ -266,7 +295,7 @@ LSMHIP_EXPORT void lsm3dcomputereinitializationeqnrhs_(real_t* out, LSMHIP_BOXP(
   if (use0 && !boxes_ok(who, fb, {{p0g, "phi0"}})) return;
   real_t h = *dx > *dy ? *dx : *dy;
   h = h > *dz ? h : *dz;
-  hipLaunchKernelGGL(reinit_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(out, rg), view(phi, pg),
+  hipLaunchKernelGGL(reinit_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(out, rg), view(phi, pg),
                      view(phi0, p0g), six(pxp, pyp, pzp, gp, pxm, pym, pzm, gm), h * h, use0);
   finish(who);
 }

@@ -127,20 +127,39 @@ inline dim3 grid_for(const Box& b) {
 }
 inline dim3 block2d() { return dim3(kBX, kBY, 1); }
 
-// Two cells along k per thread, (i, j, k0) and (i, j, k0 + 1) (nk of them inside
-// the box), on the same tiles and XCD slabs as LSMHIP_CELL: for stencil
-// kernels that issue the loads of both cells before either store, so each
-// wave keeps twice the bytes in flight.
-inline dim3 grid_for_pair(const Box& b) {
+// NK cells along k per thread, (i, j, k0) .. (i, j, k0 + NK - 1) (nk of them
+// inside the box), on the same tiles and XCD slabs as LSMHIP_CELL: for
+// kernels that issue the loads of all their cells before any store, so each
+// wave keeps NK times the bytes in flight.  The stencils take NK = 2; the
+// pointwise kernels take kPointNK, two cells in FP32 and one in FP64 (where
+// one cell already keeps enough bytes in flight, and two measured slower).
+constexpr int kPointNK = sizeof(real_t) == 4 ? 2 : 1;
+inline dim3 grid_for_cells(const Box& b, int NK) {
   const dim3 g = grid_for(b);
-  return dim3(g.x, g.y, (unsigned)((b.nz() + 1) / 2));
+  return dim3(g.x, g.y, (unsigned)((b.nz() + NK - 1) / NK));
 }
-#define LSMHIP_CELL_PAIR(b)                                                                       \
+inline dim3 grid_for_pair(const Box& b) { return grid_for_cells(b, 2); }
+#define LSMHIP_CELLS(b, NK)                                                                       \
   const int i = (b).ilo + (int)(blockIdx.x / kXcds) * kBX + (int)threadIdx.x;                    \
   const int j = (b).jlo + (int)((blockIdx.x % kXcds) * gridDim.y + blockIdx.y) * kBY + (int)threadIdx.y; \
-  const int k0 = (b).klo + 2 * (int)blockIdx.z;                                                   \
-  const int nk = (b).khi - k0 + 1 < 2 ? (b).khi - k0 + 1 : 2;                                     \
+  const int k0 = (b).klo + (NK) * (int)blockIdx.z;                                                \
+  const int nk = (b).khi - k0 + 1 < (NK) ? (b).khi - k0 + 1 : (NK);                               \
   if (i > (b).ihi || j > (b).jhi) return
+#define LSMHIP_CELL_PAIR(b) LSMHIP_CELLS(b, 2)
+
+// The cells of LSMHIP_CELLS (nk <= 2): load(k) for each (every load issued
+// before any store), then store(k, loaded) for each.
+template <class L, class S>
+__device__ __forceinline__ void pair_cells(int k0, int nk, const L& load, const S& store) {
+  const auto a = load(k0);
+  if (nk > 1) {
+    const auto b = load(k0 + 1);
+    store(k0, a);
+    store(k0 + 1, b);
+  } else {
+    store(k0, a);
+  }
+}
 
 #define LSMHIP_CELL(b)                                                                            \
   const int i = (b).ilo + (int)(blockIdx.x / kXcds) * kBX + (int)threadIdx.x;                    \
