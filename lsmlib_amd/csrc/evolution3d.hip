@@ -113,26 +113,53 @@ struct CurvFactors {
   real_t xx, yy, zz, xy, xz, yz;
 };
 
-__device__ __forceinline__ real_t mean_curvature_cell(const View<const real_t>& F, int i, int j, int k,
-                                                      const CurvFactors& f, real_t x, real_t y, real_t z, real_t gm) {
-  const real_t c = F(i, j, k);
-  const real_t xx = (F(i + 1, j, k) - real_t(2) * c + F(i - 1, j, k)) * f.xx;
-  const real_t yy = (F(i, j + 1, k) - real_t(2) * c + F(i, j - 1, k)) * f.yy;
-  const real_t zz = (F(i, j, k + 1) - real_t(2) * c + F(i, j, k - 1)) * f.zz;
-  const real_t xy = (F(i + 1, j + 1, k) - F(i + 1, j - 1, k) - F(i - 1, j + 1, k) + F(i - 1, j - 1, k)) * f.xy;
-  const real_t xz = (F(i + 1, j, k + 1) - F(i + 1, j, k - 1) - F(i - 1, j, k + 1) + F(i - 1, j, k - 1)) * f.xz;
-  const real_t yz = (F(i, j + 1, k + 1) - F(i, j + 1, k - 1) - F(i, j - 1, k + 1) + F(i, j - 1, k - 1)) * f.yz;
-  const real_t den = gm * gm * gm;
+// Second-order mean curvature (src/geometry/lsm_curvature3d.f.in:15-128): the inputs at
+// one cell, loaded before any is used.
+struct CurvInputs {
+  real_t c, xp, xm, yp, ym, zp, zm;  // F at the cell and its 6 neighbours
+  real_t xy[4], xz[4], yz[4];        // F at (+,+), (+,-), (-,+), (-,-) of each plane pair
+  real_t x, y, z, gm;
+};
+__device__ __forceinline__ CurvInputs curv_inputs(const View<const real_t>& F, int i, int j, int k,
+                                                  const View<const real_t>& px, const View<const real_t>& py,
+                                                  const View<const real_t>& pz, const View<const real_t>& gmag) {
+  return CurvInputs{F(i, j, k),
+                    F(i + 1, j, k),
+                    F(i - 1, j, k),
+                    F(i, j + 1, k),
+                    F(i, j - 1, k),
+                    F(i, j, k + 1),
+                    F(i, j, k - 1),
+                    {F(i + 1, j + 1, k), F(i + 1, j - 1, k), F(i - 1, j + 1, k), F(i - 1, j - 1, k)},
+                    {F(i + 1, j, k + 1), F(i + 1, j, k - 1), F(i - 1, j, k + 1), F(i - 1, j, k - 1)},
+                    {F(i, j + 1, k + 1), F(i, j + 1, k - 1), F(i, j - 1, k + 1), F(i, j - 1, k - 1)},
+                    px(i, j, k),
+                    py(i, j, k),
+                    pz(i, j, k),
+                    gmag(i, j, k)};
+}
+// Compact second differences of F, numerator over |grad phi|^3 (zero when that
+// is below zero_tol), in the reference's operation order.
+__device__ __forceinline__ real_t mean_curvature_of(const CurvInputs& a, const CurvFactors& f) {
+  const real_t xx = (a.xp - real_t(2) * a.c + a.xm) * f.xx;
+  const real_t yy = (a.yp - real_t(2) * a.c + a.ym) * f.yy;
+  const real_t zz = (a.zp - real_t(2) * a.c + a.zm) * f.zz;
+  const real_t xy = (a.xy[0] - a.xy[1] - a.xy[2] + a.xy[3]) * f.xy;
+  const real_t xz = (a.xz[0] - a.xz[1] - a.xz[2] + a.xz[3]) * f.xz;
+  const real_t yz = (a.yz[0] - a.yz[1] - a.yz[2] + a.yz[3]) * f.yz;
+  const real_t den = a.gm * a.gm * a.gm;
   if (den < kZeroTol) return real_t(0);
-  return stencil::curv_num(xx, xy, xz, yy, yz, zz, x, y, z) / den;
+  return stencil::curv_num(xx, xy, xz, yy, yz, zz, a.x, a.y, a.z) / den;
 }
 
 __global__ void __launch_bounds__(256) mean_curv_kernel(Box fb, View<real_t> kappa, View<const real_t> phi,
                                                         View<const real_t> px, View<const real_t> py,
                                                         View<const real_t> pz, View<const real_t> gmag,
                                                         CurvFactors f) {
-  LSMHIP_CELL(fb);
-  kappa(i, j, k) = mean_curvature_cell(phi, i, j, k, f, px(i, j, k), py(i, j, k), pz(i, j, k), gmag(i, j, k));
+  LSMHIP_CELLS(fb, kPointNK);
+  pair_cells(
+      k0, nk, [&](int k) { return curv_inputs(phi, i, j, k, px, py, pz, gmag); },
+      [&](int k, const CurvInputs& a) { kappa(i, j, k) = mean_curvature_of(a, f); });
 }
 
 // RK stage kinds: 1 = u + dt L;  2 = 0.5 (u + u1 + dt L);  32 = 0.75 u + 0.25 (u1 + dt L);
@@ -313,7 +340,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2_(real_t* kappa, LSMHIP_BOXP(k
     return;
   CurvFactors f{real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
                 real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz};
-  hipLaunchKernelGGL(mean_curv_kernel, grid_for(fb), block2d(), 0, stream(), fb, view(kappa, kg), view(phi, pg),
+  hipLaunchKernelGGL(mean_curv_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(kappa, kg), view(phi, pg),
                      view(px, gg), view(py, gg), view(pz, gg), view(gmag, gg), f);
   finish(who);
 }
