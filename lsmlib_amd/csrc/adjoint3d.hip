@@ -282,6 +282,7 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_normal_adjoint(real_t* lam_us, real_t* lam_
                                                     LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, real_t vn,
                                                     const real_t* vn_field, real_t dt, real_t dx, real_t dy,
                                                     real_t dz, real_t* scratch) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, lam_us, lam_out, phi_stage, scratch);
   const char* who = "lsmhip_weno5_stage_normal_adjoint";
   AdjArgs A{};
   A.gb = LSMHIP_BOX(gb);
@@ -324,6 +325,7 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_normal_adjoint(real_t* lam_us, real_t* lam_
 // Adjoint of lsmhip_fill_ghosts(kind = periodic, all faces): adds every ghost
 // entry of lam to the interior cell it copies and zeroes the ghosts.
 LSMHIP_EXPORT int lsmhip_periodic_fill_adjoint(real_t* lam, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, lam);
   const char* who = "lsmhip_periodic_fill_adjoint";
   Box g = LSMHIP_BOX(gb), f = LSMHIP_BOX(fb);
   if (f.empty()) return 0;
@@ -344,6 +346,7 @@ LSMHIP_EXPORT int lsmhip_periodic_fill_adjoint(real_t* lam, LSMHIP_BOXP(gb), LSM
 LSMHIP_EXPORT int lsmhip_heaviside_mismatch(double* loss, real_t* grad, const real_t* phi, const real_t* target,
                                             LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), double dx, double dy, double dz,
                                             double epsilon) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, loss, phi, target);
   const char* who = "lsmhip_heaviside_mismatch";
   Box g = LSMHIP_BOX(gb), f = LSMHIP_BOX(fb);
   if (!check_box(g, f, who, "phi") || !check_size(g, who)) return -1;

@@ -508,6 +508,7 @@ LSMHIP_EXPORT void lsm3ddeterminenarrowband_(const real_t* phi, LSMHIP_BOXP(g), 
                                              int* nlo_outer_plus, int* nhi_outer_plus, int* nlo_outer_minus,
                                              int* nhi_outer_minus, const real_t* width, const real_t* width_inner,
                                              const int* level) {
+  LSMHIP_REQUIRE_ARRAYS(phi, nb, ix, iy, iz, outer);
   const char* who = "lsm3ddeterminenarrowband_";
   const Box B = LSMHIP_BOX(g), nbb = LSMHIP_BOX(nbb);
   if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
@@ -546,6 +547,7 @@ LSMHIP_EXPORT void lsm3ddeterminenarrowbandfrommask_(const real_t* mask, LSMHIP_
                                                      LSMHIP_BOXP(nbb), int* ix, int* iy, int* iz, const int* nlo,
                                                      const int* nhi, int* n_lo, int* n_hi, const int* level,
                                                      const int* use_mask_sign) {
+  LSMHIP_REQUIRE_ARRAYS(mask, nb, ix, iy, iz);
   const char* who = "lsm3ddeterminenarrowbandfrommask_";
   const Box B = LSMHIP_BOX(mg), nbb = LSMHIP_BOX(nbb);
   if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
@@ -607,6 +609,7 @@ static void mark_boundary(const char* who, unsigned char* nb, const Box& gb, con
 // :303-377 -- six fill-box planes, each spanning the ghost box of the other axes.
 LSMHIP_EXPORT void lsm3dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb),
                                                      const unsigned char* mark) {
+  LSMHIP_REQUIRE_ARRAYS(nb);
   mark_boundary("lsm3dmarknarrowbandboundarylayer_", nb, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *mark, 3);
 }
 
@@ -666,6 +669,7 @@ LSMHIP_EXPORT void lsm3dcheckouternarrowbandlayer_(int* change_sign, const real_
                                                    const int* outer, const int* nlo_outer, const int* nhi_outer,
                                                    const int* nlo_plus, const int* nhi_plus, const int* nlo_minus,
                                                    const int* nhi_minus) {
+  LSMHIP_REQUIRE_ARRAYS(phi, ix, iy, iz, outer);
   check_outer("lsm3dcheckouternarrowbandlayer_", change_sign, phi, LSMHIP_BOX(g), ix, iy, iz, nlo, nhi, outer,
               nlo_outer, nhi_outer, nlo_plus, nhi_plus, nlo_minus, nhi_minus);
 }
@@ -686,6 +690,7 @@ LSMHIP_EXPORT void lsm2ddeterminenarrowband_(const real_t* phi, LSMHIP_BOXP2(g),
                                              int* nlo_outer_plus, int* nhi_outer_plus, int* nlo_outer_minus,
                                              int* nhi_outer_minus, const real_t* width, const real_t* width_inner,
                                              const int* level) {
+  LSMHIP_REQUIRE_ARRAYS(phi, nb, ix, iy, outer);
   const char* who = "lsm2ddeterminenarrowband_";
   const Box B = LSMHIP_BOX2(g), nbb = LSMHIP_BOX2(nbb);
   if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
@@ -717,6 +722,7 @@ LSMHIP_EXPORT void lsm2ddeterminenarrowbandfrommask_(const real_t* mask, LSMHIP_
                                                      LSMHIP_BOXP2(nbb), int* ix, int* iy, const int* nlo,
                                                      const int* nhi, int* n_lo, int* n_hi, const int* level,
                                                      const int* use_mask_sign) {
+  LSMHIP_REQUIRE_ARRAYS(mask, nb, ix, iy);
   const char* who = "lsm2ddeterminenarrowbandfrommask_";
   const Box B = LSMHIP_BOX2(mg), nbb = LSMHIP_BOX2(nbb);
   if (!check_box(nbb, B, who, "narrow_band") || !check_size(B, who) || !check_size(nbb, who)) return;
@@ -741,6 +747,7 @@ LSMHIP_EXPORT void lsm2ddeterminenarrowbandfrommask_(const real_t* mask, LSMHIP_
 // :486-533 -- the four fill-box lines, each spanning the ghost box.
 LSMHIP_EXPORT void lsm2dmarknarrowbandboundarylayer_(unsigned char* nb, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb),
                                                      const unsigned char* mark) {
+  LSMHIP_REQUIRE_ARRAYS(nb);
   mark_boundary("lsm2dmarknarrowbandboundarylayer_", nb, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *mark, 2);
 }
 
@@ -749,6 +756,7 @@ LSMHIP_EXPORT void lsm2dcheckouternarrowbandlayer_(int* change_sign, const real_
                                                    const int* iy, const int* nlo, const int* nhi, const int* outer,
                                                    const int* nlo_outer, const int* nhi_outer, const int* nlo_plus,
                                                    const int* nhi_plus, const int* nlo_minus, const int* nhi_minus) {
+  LSMHIP_REQUIRE_ARRAYS(phi, ix, iy, outer);
   check_outer("lsm2dcheckouternarrowbandlayer_", change_sign, phi, LSMHIP_BOX2(g), ix, iy, nullptr, nlo, nhi, outer,
               nlo_outer, nhi_outer, nlo_plus, nhi_plus, nlo_minus, nhi_minus);
 }

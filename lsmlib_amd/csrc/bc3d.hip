@@ -201,32 +201,40 @@ int shell(const char* who, real_t* phi, const Box& gb, const Box& fb, int kind) 
 }  // namespace
 
 LSMHIP_EXPORT void lsm3dlinearextrapolation_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dlinearextrapolation_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kLinear);
 }
 LSMHIP_EXPORT void lsm3dsignedlinearextrapolation_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dsignedlinearextrapolation_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kSigned);
 }
 LSMHIP_EXPORT void lsm3dcopyextrapolation_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dcopyextrapolation_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kCopy);
 }
 // Homogeneous Neumann fills of every scheme are copy extrapolation
 // (lsm_boundary_conditions3d.f:486-638).
 LSMHIP_EXPORT void lsm3dhomogeneousneumanneno1_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dhomogeneousneumanneno1_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm3dhomogeneousneumanneno2_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dhomogeneousneumanneno2_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm3dhomogeneousneumanneno3_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dhomogeneousneumanneno3_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm3dhomogeneousneumannweno5_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsm3dhomogeneousneumannweno5_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kCopy);
 }
 // Periodic fill of one face (no reference counterpart: the serial package has
 // no periodic boundary condition).  Applied to faces 0..5 in order it fills
 // edges and corners periodically too.
 LSMHIP_EXPORT void lsmhip_periodic_face_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face("lsmhip_periodic_face_", phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), *idx, kPeriodic);
 }
 
@@ -236,6 +244,7 @@ LSMHIP_EXPORT void lsmhip_periodic_face_(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BO
 // the order x_lo, x_hi, y_lo, y_hi, z_lo, z_hi.  kind: 0 linear, 1 signed
 // linear, 2 copy (= homogeneous Neumann), 3 periodic.
 LSMHIP_EXPORT int lsmhip_fill_ghosts(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int bdry_location_idx, int kind) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi);
   static const char* who = "lsmhip_fill_ghosts";
   if (kind < 0 || kind > 3) {
     set_error(kErrArg, "%s: unknown fill kind %d", who, kind);
@@ -256,4 +265,61 @@ LSMHIP_EXPORT int lsmhip_fill_ghosts(real_t* phi, LSMHIP_BOXP(gb), LSMHIP_BOXP(f
   if (bdry_location_idx == 9) return shell(who, phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), kind);
   for (int f = first; f <= last; f++) face(who, phi, LSMHIP_BOX(gb), LSMHIP_BOX(fb), f, kind);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's ghost-fill dispatch over a Grid, under its own names
+// (src/boundary_conditions/lsm_boundary_conditions.c:19-886).  A 3D grid takes
+// lsmhip_fill_ghosts on the grid's ghost and fill boxes (the same face groups
+// and order); a 2D grid takes the 2D face fills one after the other.  Location
+// indices the reference's switch does not list do nothing, as there.
+// ---------------------------------------------------------------------------
+namespace {
+void grid_fill(const char* who, real_t* phi, const Grid* g, int idx, int kind) {
+  if (!g) {
+    set_error(kErrArg, "%s: grid is NULL (nothing was done)", who);
+    return;
+  }
+  if (!phi) {
+    set_error(kErrArg, "%s: array argument phi is NULL (nothing was done)", who);
+    return;
+  }
+  if (g->num_dims == 3) {
+    if (idx >= 0 && idx <= 9)
+      lsmhip_fill_ghosts(phi, &g->ilo_gb, &g->ihi_gb, &g->jlo_gb, &g->jhi_gb, &g->klo_gb, &g->khi_gb, &g->ilo_fb,
+                         &g->ihi_fb, &g->jlo_fb, &g->jhi_fb, &g->klo_fb, &g->khi_fb, idx, kind);
+    return;
+  }
+  if (g->num_dims != 2) return;
+  // 2D: faces 0..3, groups 6 (x), 7 (y) and 9 (all four); 4, 5 and 8 are not in the reference's 2D switch
+  int first = 0, last = -1;
+  if (idx >= 0 && idx <= 3) first = last = idx;
+  else if (idx == 6 || idx == 7) {
+    first = 2 * (idx - 6);
+    last = first + 1;
+  } else if (idx == 9) {
+    first = 0;
+    last = 3;
+  }
+  void (*const fill[3])(real_t*, const int*, const int*, const int*, const int*, const int*, const int*, const int*,
+                        const int*, const int*) = {lsm2dlinearextrapolation_, lsm2dsignedlinearextrapolation_,
+                                                   lsm2dcopyextrapolation_};
+  for (int f = first; f <= last; f++)
+    fill[kind](phi, &g->ilo_gb, &g->ihi_gb, &g->jlo_gb, &g->jhi_gb, &g->ilo_fb, &g->ihi_fb, &g->jlo_fb, &g->jhi_fb,
+               &f);
+}
+}  // namespace
+
+LSMHIP_EXPORT void linearExtrapolationBC(real_t* phi, Grid* grid, int bdry_location_idx) {
+  grid_fill("linearExtrapolationBC", phi, grid, bdry_location_idx, kLinear);
+}
+LSMHIP_EXPORT void signedLinearExtrapolationBC(real_t* phi, Grid* grid, int bdry_location_idx) {
+  grid_fill("signedLinearExtrapolationBC", phi, grid, bdry_location_idx, kSigned);
+}
+LSMHIP_EXPORT void copyExtrapolationBC(real_t* phi, Grid* grid, int bdry_location_idx) {
+  grid_fill("copyExtrapolationBC", phi, grid, bdry_location_idx, kCopy);
+}
+// homogeneousNeumannBC is copy extrapolation (lsm_boundary_conditions.c:880-886)
+LSMHIP_EXPORT void homogeneousNeumannBC(real_t* phi, Grid* grid, int bdry_location_idx) {
+  grid_fill("homogeneousNeumannBC", phi, grid, bdry_location_idx, kCopy);
 }

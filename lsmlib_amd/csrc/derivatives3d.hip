@@ -210,19 +210,23 @@ void hj_upwind(const char* who, real_t* px, real_t* py, real_t* pz, const Box& g
 
 LSMHIP_EXPORT void lsm3dhjeno1_(PM_ARGS, real_t*, LSMHIP_BOXP(d1), LSMHIP_BOXP(fb), const real_t* dx,
                                 const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi);
   hj_plus_minus<1>("lsm3dhjeno1_", PM_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dhjeno2_(PM_ARGS, real_t*, LSMHIP_BOXP(d1), real_t*, LSMHIP_BOXP(d2), LSMHIP_BOXP(fb),
                                 const real_t* dx, const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi);
   hj_plus_minus<2>("lsm3dhjeno2_", PM_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dhjeno3_(PM_ARGS, real_t*, LSMHIP_BOXP(d1), real_t*, LSMHIP_BOXP(d2), real_t*,
                                 LSMHIP_BOXP(d3), LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                 const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi);
   hj_plus_minus<3>("lsm3dhjeno3_", PM_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dhjweno5_(PM_ARGS, real_t*, LSMHIP_BOXP(d1), LSMHIP_BOXP(fb), const real_t* dx,
                                  const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi);
   hj_plus_minus<5>("lsm3dhjweno5_", PM_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 
@@ -233,25 +237,30 @@ LSMHIP_EXPORT void lsm3dhjweno5_(PM_ARGS, real_t*, LSMHIP_BOXP(d1), LSMHIP_BOXP(
 
 LSMHIP_EXPORT void lsm3dupwindhjeno1_(UP_ARGS, real_t*, LSMHIP_BOXP(d1), LSMHIP_BOXP(fb), const real_t* dx,
                                       const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, vx, vy, vz);
   hj_upwind<1>("lsm3dupwindhjeno1_", UP_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dupwindhjeno2_(UP_ARGS, real_t*, LSMHIP_BOXP(d1), real_t*, LSMHIP_BOXP(d2), LSMHIP_BOXP(fb),
                                       const real_t* dx, const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, vx, vy, vz);
   hj_upwind<2>("lsm3dupwindhjeno2_", UP_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dupwindhjeno3_(UP_ARGS, real_t*, LSMHIP_BOXP(d1), real_t*, LSMHIP_BOXP(d2), real_t*,
                                       LSMHIP_BOXP(d3), LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                       const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, vx, vy, vz);
   hj_upwind<3>("lsm3dupwindhjeno3_", UP_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 LSMHIP_EXPORT void lsm3dupwindhjweno5_(UP_ARGS, real_t*, LSMHIP_BOXP(d1), LSMHIP_BOXP(fb), const real_t* dx,
                                        const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, vx, vy, vz);
   hj_upwind<5>("lsm3dupwindhjweno5_", UP_PASS, LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
 
 LSMHIP_EXPORT void lsm3dcentralgradorder2_(real_t* px, real_t* py, real_t* pz, LSMHIP_BOXP(g), const real_t* phi,
                                            LSMHIP_BOXP(pg), LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                            const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi);
   const char* who = "lsm3dcentralgradorder2_";
   Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g), pg = LSMHIP_BOX(pg);
   if (fb.empty()) return;
@@ -266,6 +275,7 @@ LSMHIP_EXPORT void lsm3dcentralgradorder2_(real_t* px, real_t* py, real_t* pz, L
 LSMHIP_EXPORT void lsm3dcentralgradorder4_(real_t* px, real_t* py, real_t* pz, LSMHIP_BOXP(g), const real_t* phi,
                                            LSMHIP_BOXP(pg), LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                            const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi);
   const char* who = "lsm3dcentralgradorder4_";
   Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g), pg = LSMHIP_BOX(pg);
   if (fb.empty()) return;
@@ -280,6 +290,7 @@ LSMHIP_EXPORT void lsm3dcentralgradorder4_(real_t* px, real_t* py, real_t* pz, L
 
 LSMHIP_EXPORT void lsm3dlaplacianorder2_(real_t* lap, LSMHIP_BOXP(lg), const real_t* phi, LSMHIP_BOXP(pg),
                                          LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(lap, phi);
   const char* who = "lsm3dlaplacianorder2_";
   Box fb = LSMHIP_BOX(fb), lg = LSMHIP_BOX(lg), pg = LSMHIP_BOX(pg);
   if (fb.empty()) return;
@@ -294,6 +305,7 @@ LSMHIP_EXPORT void lsm3dlaplacianorder2_(real_t* lap, LSMHIP_BOXP(lg), const rea
 LSMHIP_EXPORT void lsm3daveragegradphi_(real_t* px, real_t* py, real_t* pz, LSMHIP_BOXP(g), real_t* pxp,
                                         real_t* pyp, real_t* pzp, LSMHIP_BOXP(gp), real_t* pxm, real_t* pym,
                                         real_t* pzm, LSMHIP_BOXP(gm), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, pxp, pyp, pzp, pxm, pym, pzm);
   const char* who = "lsm3daveragegradphi_";
   Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm);
   if (fb.empty()) return;
@@ -308,6 +320,7 @@ LSMHIP_EXPORT void lsm3daveragegradphi_(real_t* px, real_t* py, real_t* pz, LSMH
 
 LSMHIP_EXPORT void lsm3dgradientmagnitude_(const real_t* px, const real_t* py, const real_t* pz, real_t* mag,
                                            LSMHIP_BOXP(g), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, mag);
   const char* who = "lsm3dgradientmagnitude_";
   Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g);
   if (fb.empty()) return;

@@ -204,6 +204,7 @@ void rk_stage(const char* who, real_t* out, const Box& og, const real_t* us, con
 // RHS terms (src/toolbox/lsm_level_set_evolution3d.h)
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm3dzerooutlevelseteqnrhs_(real_t* rhs, LSMHIP_BOXP(rg)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs);
   const char* who = "lsm3dzerooutlevelseteqnrhs_";
   Box rg = LSMHIP_BOX(rg);
   if (rg.empty() || !check_size(rg, who)) return;
@@ -215,6 +216,7 @@ LSMHIP_EXPORT void lsm3daddadvectiontermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), 
                                                   const real_t* pz, LSMHIP_BOXP(g), const real_t* vx,
                                                   const real_t* vy, const real_t* vz, LSMHIP_BOXP(vg),
                                                   LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, pz, vx, vy, vz);
   const char* who = "lsm3daddadvectiontermtolserhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), g = LSMHIP_BOX(g), vg = LSMHIP_BOX(vg);
   if (fb.empty() || !boxes_ok(who, fb, {{rg, "lse_rhs"}, {g, "grad_phi"}, {vg, "vel"}})) return;
@@ -228,6 +230,7 @@ LSMHIP_EXPORT void lsm3daddnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), 
                                                   const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                   LSMHIP_BOXP(gm), const real_t* vn, LSMHIP_BOXP(vg),
                                                   LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, vn);
   const char* who = "lsm3daddnormalveltermtolserhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm), vg = LSMHIP_BOX(vg);
   if (fb.empty() ||
@@ -242,6 +245,7 @@ LSMHIP_EXPORT void lsm3daddconstnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP(
                                                        const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
                                                        const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                        LSMHIP_BOXP(gm), const real_t* vel_n, LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm);
   const char* who = "lsm3daddconstnormalveltermtolserhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm);
   if (fb.empty() || !(fabs(*vel_n) >= kZeroTol)) return;
@@ -257,6 +261,7 @@ LSMHIP_EXPORT void lsm3daddconstcurvtermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), 
                                                   const real_t* xy, const real_t* xz, const real_t* yy,
                                                   const real_t* yz, const real_t* zz, LSMHIP_BOXP(g2),
                                                   const real_t* b, LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, pz, xx, xy, xz, yy, yz, zz);
   const char* who = "lsm3daddconstcurvtermtolserhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), g = LSMHIP_BOX(g), g2 = LSMHIP_BOX(g2);
   if (fb.empty() || !boxes_ok(who, fb, {{rg, "lse_rhs"}, {g, "grad_phi"}, {g2, "grad2_phi"}})) return;
@@ -276,6 +281,7 @@ static Box quirky_loop_box(const Box& rg, const Box& fb) {
 LSMHIP_EXPORT void lsm3daddconstprecomputedcurvtermtolserhs_(real_t* rhs, LSMHIP_BOXP(rg), const real_t* kappa,
                                                              LSMHIP_BOXP(kg), const real_t* gmag, LSMHIP_BOXP(gg),
                                                              const real_t* b, LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, kappa, gmag);
   const char* who = "lsm3daddconstprecomputedcurvtermtolserhs_";
   Box rg = LSMHIP_BOX(rg), kg = LSMHIP_BOX(kg), gg = LSMHIP_BOX(gg);
   Box lb = quirky_loop_box(rg, LSMHIP_BOX(fb));
@@ -291,6 +297,7 @@ LSMHIP_EXPORT void lsm3daddexternalandnormalveltermtolserhs_(real_t* rhs, LSMHIP
                                                              LSMHIP_BOXP(gm), const real_t* vn, const real_t* vx,
                                                              const real_t* vy, const real_t* vz, LSMHIP_BOXP(vg),
                                                              LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, vn, vx, vy, vz);
   const char* who = "lsm3daddexternalandnormalveltermtolserhs_";
   Box rg = LSMHIP_BOX(rg), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm), vg = LSMHIP_BOX(vg);
   Box lb = quirky_loop_box(rg, LSMHIP_BOX(fb));
@@ -313,6 +320,8 @@ LSMHIP_EXPORT void lsm3dcomputereinitializationeqnrhs_(real_t* out, LSMHIP_BOXP(
                                                        const real_t* pzm, LSMHIP_BOXP(gm), LSMHIP_BOXP(fb),
                                                        const real_t* dx, const real_t* dy, const real_t* dz,
                                                        const int* use_phi0_for_sgn) {
+  LSMHIP_REQUIRE_ARRAYS(out, phi, pxp, pyp, pzp, pxm, pym, pzm);
+  if (*use_phi0_for_sgn == 1) LSMHIP_REQUIRE_ARRAYS(phi0);
   const char* who = "lsm3dcomputereinitializationeqnrhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), pg = LSMHIP_BOX(pg), p0g = LSMHIP_BOX(p0g), gp = LSMHIP_BOX(gp),
       gm = LSMHIP_BOX(gm);
@@ -332,6 +341,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2_(real_t* kappa, LSMHIP_BOXP(k
                                                     const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
                                                     LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                                     const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag);
   const char* who = "lsm3dcomputemeancurvatureorder2_";
   Box fb = LSMHIP_BOX(fb), kg = LSMHIP_BOX(kg), pg = LSMHIP_BOX(pg), gg = LSMHIP_BOX(gg);
   if (fb.empty()) return;
@@ -350,34 +360,40 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2_(real_t* kappa, LSMHIP_BOXP(k
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm3drk1step_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs,
                                  LSMHIP_BOXP(rg), LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk_stage<1>("lsm3drk1step_", un, LSMHIP_BOX(ng), nullptr, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), rhs, LSMHIP_BOX(rg),
               LSMHIP_BOX(fb), *dt);
 }
 LSMHIP_EXPORT void lsm3dtvdrk2stage1_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg),
                                       const real_t* rhs, LSMHIP_BOXP(rg), LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk_stage<1>("lsm3dtvdrk2stage1_", un, LSMHIP_BOX(ng), nullptr, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), rhs,
               LSMHIP_BOX(rg), LSMHIP_BOX(fb), *dt);
 }
 LSMHIP_EXPORT void lsm3dtvdrk2stage2_(real_t* un, LSMHIP_BOXP(ng), const real_t* u1, LSMHIP_BOXP(g1),
                                       const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                       LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, u1, uc, rhs);
   rk_stage<2>("lsm3dtvdrk2stage2_", un, LSMHIP_BOX(ng), u1, LSMHIP_BOX(g1), uc, LSMHIP_BOX(cg), rhs, LSMHIP_BOX(rg),
               LSMHIP_BOX(fb), *dt);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage1_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg),
                                       const real_t* rhs, LSMHIP_BOXP(rg), LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk_stage<1>("lsm3dtvdrk3stage1_", un, LSMHIP_BOX(ng), nullptr, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), rhs,
               LSMHIP_BOX(rg), LSMHIP_BOX(fb), *dt);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage2_(real_t* u2, LSMHIP_BOXP(g2), const real_t* u1, LSMHIP_BOXP(g1),
                                       const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                       LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(u2, u1, uc, rhs);
   rk_stage<32>("lsm3dtvdrk3stage2_", u2, LSMHIP_BOX(g2), u1, LSMHIP_BOX(g1), uc, LSMHIP_BOX(cg), rhs,
                LSMHIP_BOX(rg), LSMHIP_BOX(fb), *dt);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage3_(real_t* un, LSMHIP_BOXP(ng), const real_t* u2, LSMHIP_BOXP(g2),
                                       const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                       LSMHIP_BOXP(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, u2, uc, rhs);
   rk_stage<33>("lsm3dtvdrk3stage3_", un, LSMHIP_BOX(ng), u2, LSMHIP_BOX(g2), uc, LSMHIP_BOX(cg), rhs,
                LSMHIP_BOX(rg), LSMHIP_BOX(fb), *dt);
 }

@@ -465,6 +465,7 @@ __global__ void __launch_bounds__(256) deform_fill_kernel(Box b, Box gb, real_t*
 LSMHIP_EXPORT int lsmhip_weno5_stage_normal(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
                                             LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, real_t vn,
                                             const real_t* vn_field, real_t dt, real_t dx, real_t dy, real_t dz) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi_out, phi_stage);
   FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
   a.rhs = vn_field ? kNormalField : kNormalConst;
   a.vn = vn;
@@ -478,6 +479,7 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_advect(real_t* phi_out, const real_t* phi_s
                                             LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, const real_t* vx,
                                             const real_t* vy, const real_t* vz, real_t dt, real_t dx, real_t dy,
                                             real_t dz) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi_out, phi_stage, vx, vy, vz);
   FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
   a.rhs = kAdvectField;
   a.vx = vx;
@@ -496,6 +498,8 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_advect_deform(real_t* phi_out, const real_t
                                                    LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage,
                                                    const real_t* const* tabs, real_t c, real_t dt, real_t dx,
                                                    real_t dy, real_t dz) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi_out, phi_stage, tabs);
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, tabs[0], tabs[1], tabs[2], tabs[3], tabs[4], tabs[5]);
   FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
   a.rhs = kAdvectDeform;
   for (int d = 0; d < 3; d++) {
@@ -510,6 +514,7 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_advect_deform(real_t* phi_out, const real_t
 LSMHIP_EXPORT int lsmhip_weno5_stage_reinit(real_t* phi_out, const real_t* phi_stage, const real_t* phi_n,
                                             LSMHIP_BOXP(gb), LSMHIP_BOXP(fb), int stage, const real_t* phi0,
                                             int use_phi0_for_sgn, real_t dt, real_t dx, real_t dy, real_t dz) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi_out, phi_stage);
   FusedArgs a = base(phi_out, phi_stage, phi_n, LSMHIP_BOX(gb), LSMHIP_BOX(fb), dt, dx, dy, dz);
   a.rhs = kReinit;
   a.phi0 = phi0;
@@ -528,6 +533,8 @@ LSMHIP_EXPORT int lsmhip_weno5_stage_reinit(real_t* phi_out, const real_t* phi_s
 // three arrays on gb over box b (for the drop-in chain and checks).
 LSMHIP_EXPORT int lsmhip_deform_velocity_fill(real_t* vx, real_t* vy, real_t* vz, LSMHIP_BOXP(gb), LSMHIP_BOXP(b),
                                               const real_t* const* tabs, real_t c) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, vx, vy, vz, tabs);
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, tabs[0], tabs[1], tabs[2], tabs[3], tabs[4], tabs[5]);
   const char* who = "lsmhip_deform_velocity_fill";
   Box gb = LSMHIP_BOX(gb), bb = LSMHIP_BOX(b);
   if (bb.empty()) return 0;

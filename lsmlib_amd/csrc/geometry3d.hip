@@ -407,6 +407,7 @@ void launch_pointwise(const Box& fb, const Op& op) {
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm3dcomputeunitnormal_(real_t* nx, real_t* ny, real_t* nz, LSMHIP_BOXP(ng), const real_t* px,
                                            const real_t* py, const real_t* pz, LSMHIP_BOXP(gg), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, nz, px, py, pz);
   const char* who = "lsm3dcomputeunitnormal_";
   Box fb = LSMHIP_BOX(fb), ng = LSMHIP_BOX(ng), gg = LSMHIP_BOX(gg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}})) return;
@@ -419,6 +420,7 @@ LSMHIP_EXPORT void lsm3dcomputesignedunitnormal_(real_t* nx, real_t* ny, real_t*
                                                  LSMHIP_BOXP(gg), const real_t* phi, LSMHIP_BOXP(pg),
                                                  LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                                  const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, nz, px, py, pz, phi);
   const char* who = "lsm3dcomputesignedunitnormal_";
   Box fb = LSMHIP_BOX(fb), ng = LSMHIP_BOX(ng), gg = LSMHIP_BOX(gg), pg = LSMHIP_BOX(pg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}, {pg, "phi"}})) return;
@@ -432,6 +434,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvature_(real_t* kappa, LSMHIP_BOXP(kg), co
                                               const real_t* pxy, const real_t* pxz, const real_t* pyy,
                                               const real_t* pyz, const real_t* pzz, LSMHIP_BOXP(hg),
                                               LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, px, py, pz, pxx, pxy, pxz, pyy, pyz, pzz);
   const char* who = "lsm3dcomputemeancurvature_";
   Box fb = LSMHIP_BOX(fb), kg = LSMHIP_BOX(kg), gg = LSMHIP_BOX(gg), hg = LSMHIP_BOX(hg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}, {gg, "grad_phi"}, {hg, "hessian_phi"}})) return;
@@ -446,6 +449,7 @@ LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder2_(real_t* kappa, LSMHIP_BO
                                                         const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
                                                         LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                                         const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag);
   const char* who = "lsm3dcomputegaussiancurvatureorder2_";
   Box fb = LSMHIP_BOX(fb), kg = LSMHIP_BOX(kg), pg = LSMHIP_BOX(pg), gg = LSMHIP_BOX(gg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}, {gg, "grad_phi"}}) ||
@@ -479,6 +483,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder4_(real_t* kappa, LSMHIP_BOXP(k
                                                     const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
                                                     LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                                     const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag);
   curvature_order4<0>("lsm3dcomputemeancurvatureorder4_", kappa, LSMHIP_BOX(kg), phi, LSMHIP_BOX(pg), px, py, pz,
                       gmag, LSMHIP_BOX(gg), LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
@@ -488,6 +493,7 @@ LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder4_(real_t* kappa, LSMHIP_BO
                                                         const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
                                                         LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy,
                                                         const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag);
   curvature_order4<1>("lsm3dcomputegaussiancurvatureorder4_", kappa, LSMHIP_BOX(kg), phi, LSMHIP_BOX(pg), px, py,
                       pz, gmag, LSMHIP_BOX(gg), LSMHIP_BOX(fb), *dx, *dy, *dz);
 }
@@ -497,6 +503,7 @@ LSMHIP_EXPORT void lsm3dcomputeorthogonalizationeqnrhs_(
     const real_t* pxm, const real_t* pym, const real_t* pzm, LSMHIP_BOXP(gm), const real_t* psi, LSMHIP_BOXP(sg),
     const real_t* qxp, const real_t* qyp, const real_t* qzp, LSMHIP_BOXP(qp), const real_t* qxm, const real_t* qym,
     const real_t* qzm, LSMHIP_BOXP(qm), LSMHIP_BOXP(fb), const real_t* dx, const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, psi, qxp, qyp, qzp, qxm, qym, qzm);
   const char* who = "lsm3dcomputeorthogonalizationeqnrhs_";
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm), sg = LSMHIP_BOX(sg),
       qp = LSMHIP_BOX(qp), qm = LSMHIP_BOX(qm);
@@ -521,6 +528,7 @@ LSMHIP_EXPORT void lsm3dcomputefieldextensioneqnrhs_(real_t* rhs, LSMHIP_BOXP(rg
                                                      const real_t* nx, const real_t* ny, const real_t* nz,
                                                      LSMHIP_BOXP(ng), LSMHIP_BOXP(fb), const real_t* dx,
                                                      const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, S, phi, sx, sy, sz, nx, ny, nz);
   const char* who = "lsm3dcomputefieldextensioneqnrhs_";
   (void)S;  // part of the reference signature; the RHS reads only its upwind gradient
   Box fb = LSMHIP_BOX(fb), rg = LSMHIP_BOX(rg), pg = LSMHIP_BOX(pg), ug = LSMHIP_BOX(ug), ng = LSMHIP_BOX(ng);

@@ -542,6 +542,7 @@ LSMHIP_EXPORT void lsm2dhjeno1local_(LPM2_ARGS, real_t* D1, LSMHIP_BOXP2(d1), co
                                      const int* ix, const int* iy, const int* nlo0, const int* nhi0, const int* nlo1,
                                      const int* nhi1, const unsigned char* nb, LSMHIP_BOXP2(nbb),
                                      const unsigned char* mark_fb, const unsigned char* mark_d1) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, ix, iy, nb);
   (void)nlo1;
   const int hiD[3] = {*nhi1, *nhi1, *nhi1};
   const unsigned char* marks[3] = {mark_d1, mark_d1, mark_d1};
@@ -557,6 +558,7 @@ LSMHIP_EXPORT void lsm2dhjeno2local_(LPM2_ARGS, real_t* D1, LSMHIP_BOXP2(d1), re
                                      const int* nlo2, const int* nhi2, const unsigned char* nb, LSMHIP_BOXP2(nbb),
                                      const unsigned char* mark_fb, const unsigned char* mark_d1,
                                      const unsigned char* mark_d2) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, D2, ix, iy, nb);
   (void)nlo1;
   (void)nlo2;
   const int hiD[3] = {*nhi2, *nhi1, *nhi1};
@@ -574,6 +576,7 @@ LSMHIP_EXPORT void lsm2dhjeno3local_(LPM2_ARGS, real_t* D1, LSMHIP_BOXP2(d1), re
                                      const int* nhi3, const unsigned char* nb, LSMHIP_BOXP2(nbb),
                                      const unsigned char* mark_fb, const unsigned char* mark_d1,
                                      const unsigned char* mark_d2, const unsigned char* mark_d3) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, D2, D3, ix, iy, nb);
   (void)nlo1;
   (void)nlo2;
   (void)nlo3;
@@ -589,6 +592,7 @@ LSMHIP_EXPORT void lsm2dhjweno5local_(LPM2_ARGS, real_t* D1, LSMHIP_BOXP2(d1), c
                                       const int* nlo1, const int* nhi1, const int* nlo2, const int* nhi2,
                                       const int* nlo3, const int* nhi3, const unsigned char* nb, LSMHIP_BOXP2(nbb),
                                       const unsigned char* mark_fb, const unsigned char* mark_d1) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, ix, iy, nb);
   (void)nlo1;
   (void)nhi1;
   (void)nlo2;
@@ -610,6 +614,7 @@ LSMHIP_EXPORT void lsm2dupwindhjeno2local_(real_t* px, real_t* py, LSMHIP_BOXP2(
                                            const int* nlo2, const int* nhi2, const unsigned char* nb,
                                            LSMHIP_BOXP2(nbb), const unsigned char* mark_fb,
                                            const unsigned char* mark_d1, const unsigned char* mark_d2) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, vx, vy, D1, D2, ix, iy, nb);
   const char* who = "lsm2dupwindhjeno2local_";
   (void)nlo1;
   (void)nlo2;
@@ -637,6 +642,7 @@ LSMHIP_EXPORT void lsm2dupwindhjeno2local_(real_t* px, real_t* py, LSMHIP_BOXP2(
 
 LSMHIP_EXPORT void lsm2dcentralgradorder2local_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi,
                                                 LSMHIP_BOXP2(pg), const real_t* dx, const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, ix, iy, nb);
   const char* who = "lsm2dcentralgradorder2local_";
   const Box g = LSMHIP_BOX2(g);
   launch_points(who, LIST2, FBMASK2,
@@ -646,6 +652,7 @@ LSMHIP_EXPORT void lsm2dcentralgradorder2local_(real_t* px, real_t* py, LSMHIP_B
 
 LSMHIP_EXPORT void lsm2dcentralgradorder4local_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi,
                                                 LSMHIP_BOXP2(pg), const real_t* dx, const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, ix, iy, nb);
   const char* who = "lsm2dcentralgradorder4local_";
   const Box g = LSMHIP_BOX2(g);
   const real_t c = real_t(0.0833333333333333333332);
@@ -655,6 +662,7 @@ LSMHIP_EXPORT void lsm2dcentralgradorder4local_(real_t* px, real_t* py, LSMHIP_B
 
 LSMHIP_EXPORT void lsm2dlaplacianorder2local_(real_t* lap, LSMHIP_BOXP2(lg), const real_t* phi, LSMHIP_BOXP2(pg),
                                               const real_t* dx, const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(lap, phi, ix, iy, nb);
   const char* who = "lsm2dlaplacianorder2local_";
   launch_points(who, LIST2, FBMASK2,
                 LaplacianOp{cv(lap, LSMHIP_BOX2(lg)), cv(phi, LSMHIP_BOX2(pg)), real_t(1.0) / *dx / *dx,
@@ -667,12 +675,14 @@ LSMHIP_EXPORT void lsm2dlaplacianorder2local_(real_t* lap, LSMHIP_BOXP2(lg), con
 // is FP64 compensated in a fixed order.
 LSMHIP_EXPORT void lsm2dcomputeavegradphilocal_(real_t* ave, const real_t* phi, LSMHIP_BOXP2(pg), const real_t* dx,
                                                 const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(phi, ix, iy, nb);
   points_mean("lsm2dcomputeavegradphilocal_", LIST2, FBMASK2,
               GradNormOp{cv(phi, LSMHIP_BOX2(pg)), real_t(0.5) / *dx, real_t(0.5) / *dy}, ave);
 }
 
 LSMHIP_EXPORT void lsm2dgradientmagnitudelocal_(const real_t* px, const real_t* py, real_t* mag, LSMHIP_BOXP2(g),
                                                 LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, mag, ix, iy, nb);
   const char* who = "lsm2dgradientmagnitudelocal_";
   const Box g = LSMHIP_BOX2(g);
   launch_points(who, LIST2, FBMASK2, GradMagOp{cv(px, g), cv(py, g), cv(mag, g)});
@@ -684,6 +694,7 @@ LSMHIP_EXPORT void lsm2dgradientmagnitudelocal_(const real_t* px, const real_t* 
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm2dzerooutlevelseteqnrhslocal_(real_t* rhs, LSMHIP_BOXP2(rg), const int* ix, const int* iy,
                                                     const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, ix, iy);
   const char* who = "lsm2dzerooutlevelseteqnrhslocal_";
   launch_points(who, LIST2, no_mask(), ZeroOp{cv(rhs, LSMHIP_BOX2(rg))});
   finish_local(who);
@@ -692,6 +703,7 @@ LSMHIP_EXPORT void lsm2dzerooutlevelseteqnrhslocal_(real_t* rhs, LSMHIP_BOXP2(rg
 LSMHIP_EXPORT void lsm2daddadvectiontermtolserhslocal_(real_t* rhs, LSMHIP_BOXP2(rg), const real_t* px,
                                                        const real_t* py, LSMHIP_BOXP2(g), const real_t* vx,
                                                        const real_t* vy, LSMHIP_BOXP2(vg), LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, vx, vy, ix, iy, nb);
   const char* who = "lsm2daddadvectiontermtolserhslocal_";
   const Box g = LSMHIP_BOX2(g), vg = LSMHIP_BOX2(vg);
   launch_points(who, LIST2, FBMASK2,
@@ -703,6 +715,7 @@ LSMHIP_EXPORT void lsm2daddnormalveltermtolserhslocal_(real_t* rhs, LSMHIP_BOXP2
                                                        const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                        const real_t* pym, LSMHIP_BOXP2(gm), const real_t* vn,
                                                        LSMHIP_BOXP2(vg), LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, vn, ix, iy, nb);
   const char* who = "lsm2daddnormalveltermtolserhslocal_";
   launch_points(who, LIST2, FBMASK2,
                 NormalOp{cv(rhs, LSMHIP_BOX2(rg)), four(pxp, pyp, LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm)),
@@ -714,6 +727,7 @@ LSMHIP_EXPORT void lsm2daddconstnormalveltermtolserhslocal_(real_t* rhs, LSMHIP_
                                                             const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                             const real_t* pym, LSMHIP_BOXP2(gm), const real_t* vn,
                                                             LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, ix, iy, nb);
   const char* who = "lsm2daddconstnormalveltermtolserhslocal_";
   const Box none{0, -1, 0, -1, 0, -1};
   launch_points(who, LIST2, FBMASK2,
@@ -726,6 +740,7 @@ LSMHIP_EXPORT void lsm2daddconstcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP2
                                                        const real_t* py, LSMHIP_BOXP2(g), const real_t* xx,
                                                        const real_t* xy, const real_t* yy, LSMHIP_BOXP2(g2),
                                                        const real_t* b, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, xx, xy, yy, ix, iy, nb);
   const char* who = "lsm2daddconstcurvtermtolserhslocal_";
   const Box g = LSMHIP_BOX2(g), g2 = LSMHIP_BOX2(g2);
   launch_points(who, LIST2, FBMASK2,
@@ -736,6 +751,7 @@ LSMHIP_EXPORT void lsm2daddconstcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP2
 LSMHIP_EXPORT void lsm2daddconstprecomputedcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP2(rg), const real_t* kappa,
                                                                   LSMHIP_BOXP2(kg), const real_t* gmag,
                                                                   LSMHIP_BOXP2(gg), const real_t* b, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, kappa, gmag, ix, iy, nb);
   const char* who = "lsm2daddconstprecomputedcurvtermtolserhslocal_";
   launch_points(who, LIST2, FBMASK2,
                 PrecompCurvOp{cv(rhs, LSMHIP_BOX2(rg)), cv(kappa, LSMHIP_BOX2(kg)), cv(gmag, LSMHIP_BOX2(gg)), *b});
@@ -748,6 +764,7 @@ LSMHIP_EXPORT void lsm2daddexternalandnormalveltermtolserhslocal_(real_t* rhs, L
                                                                   LSMHIP_BOXP2(gm), const real_t* vn,
                                                                   const real_t* vx, const real_t* vy,
                                                                   LSMHIP_BOXP2(vg), LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, vn, vx, vy, ix, iy, nb);
   const char* who = "lsm2daddexternalandnormalveltermtolserhslocal_";
   const Box vg = LSMHIP_BOX2(vg);
   launch_points(who, LIST2, FBMASK2,
@@ -768,34 +785,40 @@ static void rk_local2(const char* who, real_t* out, const Box& ob, const real_t*
 
 LSMHIP_EXPORT void lsm2drk1steplocal_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg),
                                       const real_t* rhs, LSMHIP_BOXP2(rg), const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, nb);
   rk_local2<1>("lsm2drk1steplocal_", un, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), uc, LSMHIP_BOX2(cg), rhs,
                LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
 LSMHIP_EXPORT void lsm2dtvdrk2stage1local_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg),
                                            const real_t* rhs, LSMHIP_BOXP2(rg), const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, nb);
   rk_local2<1>("lsm2dtvdrk2stage1local_", un, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), uc, LSMHIP_BOX2(cg), rhs,
                LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
 LSMHIP_EXPORT void lsm2dtvdrk2stage2local_(real_t* un, LSMHIP_BOXP2(ng), const real_t* u1, LSMHIP_BOXP2(g1),
                                            const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                            const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, u1, uc, rhs, ix, iy, nb);
   rk_local2<2>("lsm2dtvdrk2stage2local_", un, LSMHIP_BOX2(ng), u1, LSMHIP_BOX2(g1), uc, LSMHIP_BOX2(cg), rhs,
                LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage1local_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg),
                                            const real_t* rhs, LSMHIP_BOXP2(rg), const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, nb);
   rk_local2<1>("lsm2dtvdrk3stage1local_", un, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), uc, LSMHIP_BOX2(cg), rhs,
                LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage2local_(real_t* u2, LSMHIP_BOXP2(g2), const real_t* u1, LSMHIP_BOXP2(g1),
                                            const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                            const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(u2, u1, uc, rhs, ix, iy, nb);
   rk_local2<32>("lsm2dtvdrk3stage2local_", u2, LSMHIP_BOX2(g2), u1, LSMHIP_BOX2(g1), uc, LSMHIP_BOX2(cg), rhs,
                 LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage3local_(real_t* un, LSMHIP_BOXP2(ng), const real_t* u2, LSMHIP_BOXP2(g2),
                                            const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                            const real_t* dt, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, u2, uc, rhs, ix, iy, nb);
   rk_local2<33>("lsm2dtvdrk3stage3local_", un, LSMHIP_BOX2(ng), u2, LSMHIP_BOX2(g2), uc, LSMHIP_BOX2(cg), rhs,
                 LSMHIP_BOX2(rg), *dt, LIST2, FBMASK2);
 }
@@ -809,6 +832,8 @@ LSMHIP_EXPORT void lsm2dcomputereinitializationeqnrhslocal_(real_t* out, LSMHIP_
                                                             const real_t* pxm, const real_t* pym, LSMHIP_BOXP2(gm),
                                                             const real_t* dx, const real_t* dy,
                                                             const int* use_phi0_for_sgn, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(out, phi, pxp, pyp, pxm, pym, ix, iy, nb);
+  if (*use_phi0_for_sgn == 1) LSMHIP_REQUIRE_ARRAYS(phi0);
   const char* who = "lsm2dcomputereinitializationeqnrhslocal_";
   const real_t h = *dx > *dy ? *dx : *dy;
   launch_points(who, LIST2, FBMASK2,
@@ -822,6 +847,7 @@ LSMHIP_EXPORT void lsm2dcomputeorthogonalizationeqnrhslocal_(
     const real_t* pym, LSMHIP_BOXP2(gm), const real_t* psi, LSMHIP_BOXP2(sg), const real_t* qxp, const real_t* qyp,
     LSMHIP_BOXP2(qp), const real_t* qxm, const real_t* qym, LSMHIP_BOXP2(qm), const real_t* dx, const real_t* dy,
     LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, psi, qxp, qyp, qxm, qym, ix, iy, nb);
   const char* who = "lsm2dcomputeorthogonalizationeqnrhslocal_";
   const real_t h = *dx > *dy ? *dx : *dy;
   launch_points(who, LIST2, FBMASK2,
@@ -847,6 +873,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder2local_(real_t* kappa, LSMHIP_B
                                                          LSMHIP_BOXP2(pg), const real_t* px, const real_t* py,
                                                          const real_t* gmag, LSMHIP_BOXP2(gg), const real_t* dx,
                                                          const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, gmag, ix, iy, nb);
   mean_curv_local2<2>("lsm2dcomputemeancurvatureorder2local_", kappa, LSMHIP_BOX2(kg), phi, LSMHIP_BOX2(pg), px, py,
                       gmag, LSMHIP_BOX2(gg), *dx, *dy, LIST2, FBMASK2);
 }
@@ -855,6 +882,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder4local_(real_t* kappa, LSMHIP_B
                                                          LSMHIP_BOXP2(pg), const real_t* px, const real_t* py,
                                                          const real_t* gmag, LSMHIP_BOXP2(gg), const real_t* dx,
                                                          const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, gmag, ix, iy, nb);
   mean_curv_local2<4>("lsm2dcomputemeancurvatureorder4local_", kappa, LSMHIP_BOX2(kg), phi, LSMHIP_BOX2(pg), px, py,
                       gmag, LSMHIP_BOX2(gg), *dx, *dy, LIST2, FBMASK2);
 }
@@ -863,6 +891,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder4local_(real_t* kappa, LSMHIP_B
 LSMHIP_EXPORT void lsm2dcomputeunitnormallocal_(real_t* nx, real_t* ny, LSMHIP_BOXP2(ng), const real_t* px,
                                                 const real_t* py, LSMHIP_BOXP2(gg), const real_t* dx,
                                                 const real_t* dy, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, px, py, ix, iy, nb);
   const char* who = "lsm2dcomputeunitnormallocal_";
   (void)dx;
   (void)dy;
@@ -876,6 +905,7 @@ LSMHIP_EXPORT void lsm2dcomputesignedunitnormallocal_(real_t* nx, real_t* ny, LS
                                                       const real_t* py, LSMHIP_BOXP2(gg), const real_t* phi,
                                                       LSMHIP_BOXP2(pg), const real_t* dx, const real_t* dy,
                                                       LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, px, py, phi, ix, iy, nb);
   const char* who = "lsm2dcomputesignedunitnormallocal_";
   const Box ng = LSMHIP_BOX2(ng), gg = LSMHIP_BOX2(gg);
   const real_t h = *dx > *dy ? *dx : *dy;
@@ -890,6 +920,7 @@ LSMHIP_EXPORT void lsm2dcomputesignedunitnormallocal_(real_t* nx, real_t* ny, LS
 LSMHIP_EXPORT void lsm2dimposemasklocal_(real_t* dst, LSMHIP_BOXP2(dg), const real_t* src, LSMHIP_BOXP2(sg),
                                          const real_t* msk, LSMHIP_BOXP2(mg), const int* ix, const int* iy,
                                          const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(dst, src, msk, ix, iy);
   const char* who = "lsm2dimposemasklocal_";
   launch_points(who, LIST2, no_mask(),
                 ImposeMaskOp{cv(dst, LSMHIP_BOX2(dg)), cv(src, LSMHIP_BOX2(sg)), cv(msk, LSMHIP_BOX2(mg))});
@@ -898,6 +929,7 @@ LSMHIP_EXPORT void lsm2dimposemasklocal_(real_t* dst, LSMHIP_BOXP2(dg), const re
 
 LSMHIP_EXPORT void lsm2dcopydatalocal_(real_t* dst, LSMHIP_BOXP2(dg), const real_t* src, LSMHIP_BOXP2(sg),
                                        const int* ix, const int* iy, const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(dst, src, ix, iy);
   const char* who = "lsm2dcopydatalocal_";
   launch_points(who, LIST2, no_mask(), CopyOp{cv(dst, LSMHIP_BOX2(dg)), cv(src, LSMHIP_BOX2(sg))});
   finish_local(who);
@@ -908,6 +940,7 @@ LSMHIP_EXPORT void lsm2dmultiplycutofflserhslocal_(const real_t* phi, real_t* rh
                                                    const unsigned char* nb, LSMHIP_BOXP2(nbb),
                                                    const unsigned char* mark_fb, const real_t* beta,
                                                    const real_t* gamma) {
+  LSMHIP_REQUIRE_ARRAYS(phi, rhs, ix, iy, nb);
   const char* who = "lsm2dmultiplycutofflserhslocal_";
   const Box g = LSMHIP_BOX2(g);
   const real_t c1 = *gamma - real_t(3) * *beta;
@@ -923,6 +956,7 @@ LSMHIP_EXPORT void lsm2dmultiplycutofflserhslocal_(const real_t* phi, real_t* rh
 // :16-81 -- 0 when no point is admissible
 LSMHIP_EXPORT void lsm2dmaxnormdifflocal_(real_t* out, const real_t* f1, LSMHIP_BOXP2(g1), const real_t* f2,
                                           LSMHIP_BOXP2(g2), LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2, ix, iy, nb);
   const char* who = "lsm2dmaxnormdifflocal_";
   double m = 0.0;
   bool any;
@@ -935,6 +969,7 @@ LSMHIP_EXPORT void lsm2dmaxnormdifflocal_(real_t* out, const real_t* f1, LSMHIP_
 LSMHIP_EXPORT void lsm2dcomputestableadvectiondtlocal_(real_t* dt, const real_t* vx, const real_t* vy,
                                                        LSMHIP_BOXP2(vg), const real_t* dx, const real_t* dy,
                                                        const real_t* cfl, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(vx, vy, ix, iy, nb);
   const char* who = "lsm2dcomputestableadvectiondtlocal_";
   const Box vg = LSMHIP_BOX2(vg);
   double m = -1.0;
@@ -965,6 +1000,7 @@ LSMHIP_EXPORT void lsm2dcomputestablenormalveldtlocal_(real_t* dt, const real_t*
                                                        const real_t* pxm, const real_t* pym, LSMHIP_BOXP2(gm),
                                                        const real_t* dx, const real_t* dy, const real_t* cfl,
                                                        LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(vn, pxp, pyp, pxm, pym, ix, iy, nb);
   normal_dt_local2("lsm2dcomputestablenormalveldtlocal_", dt, vn, LSMHIP_BOX2(vg), real_t(0), pxp, pyp,
                    LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm), *dx, *dy, *cfl, LIST2, FBMASK2);
 }
@@ -974,6 +1010,7 @@ LSMHIP_EXPORT void lsm2dcomputestableconstnormalveldtlocal_(real_t* dt, const re
                                                             const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                             const real_t* pym, LSMHIP_BOXP2(gm), const real_t* dx,
                                                             const real_t* dy, const real_t* cfl, LOCAL2_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, ix, iy, nb);
   const Box none{0, -1, 0, -1, 0, -1};
   normal_dt_local2("lsm2dcomputestableconstnormalveldtlocal_", dt, nullptr, none, real_t(fabs(*vn)), pxp, pyp,
                    LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm), *dx, *dy, *cfl, LIST2, FBMASK2);

@@ -337,6 +337,7 @@ LSMHIP_EXPORT void lsm3dhjeno1local_(LPM_ARGS, real_t* D1, LSMHIP_BOXP(d1), cons
                                      const real_t* dz, const int* ix, const int* iy, const int* iz, const int* nlo0,
                                      const int* nhi0, const int* nlo1, const int* nhi1, const unsigned char* nb,
                                      LSMHIP_BOXP(nbb), const unsigned char* mark_fb, const unsigned char* mark_d1) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi, D1, ix, iy, iz, nb);
   (void)nlo1;
   local_derivs<1>("lsm3dhjeno1local_", pxp, pyp, pzp, LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), phi,
                   LSMHIP_BOX(pg), D1, LSMHIP_BOX(d1), nullptr, LSMHIP_BOX(d1), *dx, *dy, *dz, ix, iy, iz, *nlo0,
@@ -349,6 +350,7 @@ LSMHIP_EXPORT void lsm3dhjeno2local_(LPM_ARGS, real_t* D1, LSMHIP_BOXP(d1), real
                                      const int* nhi1, const int* nlo2, const int* nhi2, const unsigned char* nb,
                                      LSMHIP_BOXP(nbb), const unsigned char* mark_fb, const unsigned char* mark_d1,
                                      const unsigned char* mark_d2) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi, D1, D2, ix, iy, iz, nb);
   (void)nlo1;
   (void)nlo2;
   local_derivs<2>("lsm3dhjeno2local_", pxp, pyp, pzp, LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), phi,
@@ -366,6 +368,7 @@ LSMHIP_EXPORT void lsm3dhjweno5local_(LPM_ARGS, real_t* D1, LSMHIP_BOXP(d1), con
                                       const int* nlo2, const int* nhi2, const int* nlo3, const int* nhi3,
                                       const unsigned char* nb, LSMHIP_BOXP(nbb), const unsigned char* mark_fb,
                                       const unsigned char* mark_d1) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, phi, D1, ix, iy, iz, nb);
   (void)nlo1;
   (void)nhi1;
   (void)nlo2;
@@ -385,6 +388,7 @@ LSMHIP_EXPORT void lsm3dhjweno5local_(LPM_ARGS, real_t* D1, LSMHIP_BOXP(d1), con
 LSMHIP_EXPORT void lsm3dcentralgradorder2local_(real_t* px, real_t* py, real_t* pz, LSMHIP_BOXP(g), const real_t* phi,
                                                 LSMHIP_BOXP(pg), const real_t* dx, const real_t* dy, const real_t* dz,
                                                 LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, ix, iy, iz, nb);
   const char* who = "lsm3dcentralgradorder2local_";
   launch_points(who, LIST, FBMASK,
                 Central2Op{axis3(px, py, pz, LSMHIP_BOX(g)), cv(phi, LSMHIP_BOX(pg)), real_t(0.5) / *dx,
@@ -395,6 +399,7 @@ LSMHIP_EXPORT void lsm3dcentralgradorder2local_(real_t* px, real_t* py, real_t* 
 LSMHIP_EXPORT void lsm3dcentralgradorder4local_(real_t* px, real_t* py, real_t* pz, LSMHIP_BOXP(g), const real_t* phi,
                                                 LSMHIP_BOXP(pg), const real_t* dx, const real_t* dy, const real_t* dz,
                                                 LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, phi, ix, iy, iz, nb);
   const char* who = "lsm3dcentralgradorder4local_";
   const real_t c = real_t(0.0833333333333333333333);
   launch_points(who, LIST, FBMASK,
@@ -404,6 +409,7 @@ LSMHIP_EXPORT void lsm3dcentralgradorder4local_(real_t* px, real_t* py, real_t* 
 
 LSMHIP_EXPORT void lsm3dlaplacianorder2local_(real_t* lap, LSMHIP_BOXP(lg), const real_t* phi, LSMHIP_BOXP(pg),
                                               const real_t* dx, const real_t* dy, const real_t* dz, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(lap, phi, ix, iy, iz, nb);
   const char* who = "lsm3dlaplacianorder2local_";
   launch_points(who, LIST, FBMASK,
                 LaplacianOp{cv(lap, LSMHIP_BOX(lg)), cv(phi, LSMHIP_BOX(pg)), real_t(1.0) / *dx / *dx,
@@ -413,6 +419,7 @@ LSMHIP_EXPORT void lsm3dlaplacianorder2local_(real_t* lap, LSMHIP_BOXP(lg), cons
 
 LSMHIP_EXPORT void lsm3dgradientmagnitudelocal_(const real_t* px, const real_t* py, const real_t* pz, real_t* mag,
                                                 LSMHIP_BOXP(g), LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pz, mag, ix, iy, iz, nb);
   const char* who = "lsm3dgradientmagnitudelocal_";
   const Box g = LSMHIP_BOX(g);
   launch_points(who, LIST, FBMASK, GradMagOp{cv(px, g), cv(py, g), cv(pz, g), cv(mag, g)});
@@ -424,6 +431,7 @@ LSMHIP_EXPORT void lsm3dgradientmagnitudelocal_(const real_t* px, const real_t* 
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm3dzerooutlevelseteqnrhslocal_(real_t* rhs, LSMHIP_BOXP(rg), const int* ix, const int* iy,
                                                     const int* iz, const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, ix, iy, iz);
   const char* who = "lsm3dzerooutlevelseteqnrhslocal_";
   launch_points(who, LIST, no_mask(), ZeroOp{cv(rhs, LSMHIP_BOX(rg))});
   finish_local(who);
@@ -433,6 +441,7 @@ LSMHIP_EXPORT void lsm3daddadvectiontermtolserhslocal_(real_t* rhs, LSMHIP_BOXP(
                                                        const real_t* py, const real_t* pz, LSMHIP_BOXP(g),
                                                        const real_t* vx, const real_t* vy, const real_t* vz,
                                                        LSMHIP_BOXP(vg), LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, pz, vx, vy, vz, ix, iy, iz, nb);
   const char* who = "lsm3daddadvectiontermtolserhslocal_";
   const Box g = LSMHIP_BOX(g), vg = LSMHIP_BOX(vg);
   launch_points(who, LIST, FBMASK,
@@ -445,6 +454,7 @@ LSMHIP_EXPORT void lsm3daddnormalveltermtolserhslocal_(real_t* rhs, LSMHIP_BOXP(
                                                        const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                        LSMHIP_BOXP(gm), const real_t* vn, LSMHIP_BOXP(vg),
                                                        LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, vn, ix, iy, iz, nb);
   const char* who = "lsm3daddnormalveltermtolserhslocal_";
   launch_points(who, LIST, FBMASK,
                 NormalOp{cv(rhs, LSMHIP_BOX(rg)), six(pxp, pyp, pzp, LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm)),
@@ -456,6 +466,7 @@ LSMHIP_EXPORT void lsm3daddconstnormalveltermtolserhslocal_(real_t* rhs, LSMHIP_
                                                             const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
                                                             const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                             LSMHIP_BOXP(gm), const real_t* vn, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, ix, iy, iz, nb);
   const char* who = "lsm3daddconstnormalveltermtolserhslocal_";
   Box none{0, -1, 0, -1, 0, -1};
   launch_points(who, LIST, FBMASK,
@@ -469,6 +480,7 @@ LSMHIP_EXPORT void lsm3daddconstcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP(
                                                        const real_t* xx, const real_t* xy, const real_t* xz,
                                                        const real_t* yy, const real_t* yz, const real_t* zz,
                                                        LSMHIP_BOXP(g2), const real_t* b, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, pz, xx, xy, xz, yy, yz, zz, ix, iy, iz, nb);
   const char* who = "lsm3daddconstcurvtermtolserhslocal_";
   const Box g = LSMHIP_BOX(g), g2 = LSMHIP_BOX(g2);
   launch_points(who, LIST, FBMASK,
@@ -480,6 +492,7 @@ LSMHIP_EXPORT void lsm3daddconstcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP(
 LSMHIP_EXPORT void lsm3daddconstprecomputedcurvtermtolserhslocal_(real_t* rhs, LSMHIP_BOXP(rg), const real_t* kappa,
                                                                   LSMHIP_BOXP(kg), const real_t* gmag,
                                                                   LSMHIP_BOXP(gg), const real_t* b, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, kappa, gmag, ix, iy, iz, nb);
   const char* who = "lsm3daddconstprecomputedcurvtermtolserhslocal_";
   launch_points(who, LIST, FBMASK,
                 PrecompCurvOp{cv(rhs, LSMHIP_BOX(rg)), cv(kappa, LSMHIP_BOX(kg)), cv(gmag, LSMHIP_BOX(gg)), *b});
@@ -490,6 +503,7 @@ LSMHIP_EXPORT void lsm3daddexternalandnormalveltermtolserhslocal_(
     real_t* rhs, LSMHIP_BOXP(rg), const real_t* pxp, const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
     const real_t* pxm, const real_t* pym, const real_t* pzm, LSMHIP_BOXP(gm), const real_t* vn, const real_t* vx,
     const real_t* vy, const real_t* vz, LSMHIP_BOXP(vg), LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, vn, vx, vy, vz, ix, iy, iz, nb);
   const char* who = "lsm3daddexternalandnormalveltermtolserhslocal_";
   const Box vg = LSMHIP_BOX(vg);
   launch_points(who, LIST, FBMASK,
@@ -511,34 +525,40 @@ static void rk_local(const char* who, real_t* out, const Box& ob, const real_t* 
 
 LSMHIP_EXPORT void lsm3drk1steplocal_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg),
                                       const real_t* rhs, LSMHIP_BOXP(rg), const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, iz, nb);
   rk_local<1>("lsm3drk1steplocal_", un, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), uc, LSMHIP_BOX(cg), rhs, LSMHIP_BOX(rg),
               *dt, LIST, FBMASK);
 }
 LSMHIP_EXPORT void lsm3dtvdrk2stage1local_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg),
                                            const real_t* rhs, LSMHIP_BOXP(rg), const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, iz, nb);
   rk_local<1>("lsm3dtvdrk2stage1local_", un, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), uc, LSMHIP_BOX(cg), rhs,
               LSMHIP_BOX(rg), *dt, LIST, FBMASK);
 }
 LSMHIP_EXPORT void lsm3dtvdrk2stage2local_(real_t* un, LSMHIP_BOXP(ng), const real_t* u1, LSMHIP_BOXP(g1),
                                            const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                            const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, u1, uc, rhs, ix, iy, iz, nb);
   rk_local<2>("lsm3dtvdrk2stage2local_", un, LSMHIP_BOX(ng), u1, LSMHIP_BOX(g1), uc, LSMHIP_BOX(cg), rhs,
               LSMHIP_BOX(rg), *dt, LIST, FBMASK);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage1local_(real_t* un, LSMHIP_BOXP(ng), const real_t* uc, LSMHIP_BOXP(cg),
                                            const real_t* rhs, LSMHIP_BOXP(rg), const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs, ix, iy, iz, nb);
   rk_local<1>("lsm3dtvdrk3stage1local_", un, LSMHIP_BOX(ng), uc, LSMHIP_BOX(cg), uc, LSMHIP_BOX(cg), rhs,
               LSMHIP_BOX(rg), *dt, LIST, FBMASK);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage2local_(real_t* u2, LSMHIP_BOXP(g2), const real_t* u1, LSMHIP_BOXP(g1),
                                            const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                            const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(u2, u1, uc, rhs, ix, iy, iz, nb);
   rk_local<32>("lsm3dtvdrk3stage2local_", u2, LSMHIP_BOX(g2), u1, LSMHIP_BOX(g1), uc, LSMHIP_BOX(cg), rhs,
                LSMHIP_BOX(rg), *dt, LIST, FBMASK);
 }
 LSMHIP_EXPORT void lsm3dtvdrk3stage3local_(real_t* un, LSMHIP_BOXP(ng), const real_t* u2, LSMHIP_BOXP(g2),
                                            const real_t* uc, LSMHIP_BOXP(cg), const real_t* rhs, LSMHIP_BOXP(rg),
                                            const real_t* dt, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(un, u2, uc, rhs, ix, iy, iz, nb);
   rk_local<33>("lsm3dtvdrk3stage3local_", un, LSMHIP_BOX(ng), u2, LSMHIP_BOX(g2), uc, LSMHIP_BOX(cg), rhs,
                LSMHIP_BOX(rg), *dt, LIST, FBMASK);
 }
@@ -551,6 +571,8 @@ LSMHIP_EXPORT void lsm3dcomputereinitializationeqnrhslocal_(
     const real_t* pxp, const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp), const real_t* pxm, const real_t* pym,
     const real_t* pzm, LSMHIP_BOXP(gm), const real_t* dx, const real_t* dy, const real_t* dz,
     const int* use_phi0_for_sgn, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(out, phi, pxp, pyp, pzp, pxm, pym, pzm, ix, iy, iz, nb);
+  if (*use_phi0_for_sgn == 1) LSMHIP_REQUIRE_ARRAYS(phi0);
   const char* who = "lsm3dcomputereinitializationeqnrhslocal_";
   real_t h = *dx > *dy ? *dx : *dy;
   h = h > *dz ? h : *dz;
@@ -566,6 +588,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2local_(real_t* kappa, LSMHIP_B
                                                          const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
                                                          const real_t* dx, const real_t* dy, const real_t* dz,
                                                          LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag, ix, iy, iz, nb);
   const char* who = "lsm3dcomputemeancurvatureorder2local_";
   const Box gg = LSMHIP_BOX(gg);
   launch_points(who, LIST, FBMASK,
@@ -581,6 +604,7 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2local_(real_t* kappa, LSMHIP_B
 LSMHIP_EXPORT void lsm3dimposemasklocal_(real_t* dst, LSMHIP_BOXP(dg), const real_t* src, LSMHIP_BOXP(sg),
                                          const real_t* msk, LSMHIP_BOXP(mg), const int* ix, const int* iy,
                                          const int* iz, const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(dst, src, msk, ix, iy, iz);
   const char* who = "lsm3dimposemasklocal_";
   launch_points(who, LIST, no_mask(),
                 ImposeMaskOp{cv(dst, LSMHIP_BOX(dg)), cv(src, LSMHIP_BOX(sg)), cv(msk, LSMHIP_BOX(mg))});
@@ -589,6 +613,7 @@ LSMHIP_EXPORT void lsm3dimposemasklocal_(real_t* dst, LSMHIP_BOXP(dg), const rea
 
 LSMHIP_EXPORT void lsm3dcopydatalocal_(real_t* dst, LSMHIP_BOXP(dg), const real_t* src, LSMHIP_BOXP(sg),
                                        const int* ix, const int* iy, const int* iz, const int* nlo, const int* nhi) {
+  LSMHIP_REQUIRE_ARRAYS(dst, src, ix, iy, iz);
   const char* who = "lsm3dcopydatalocal_";
   launch_points(who, LIST, no_mask(), CopyOp{cv(dst, LSMHIP_BOX(dg)), cv(src, LSMHIP_BOX(sg))});
   finish_local(who);
@@ -599,6 +624,7 @@ LSMHIP_EXPORT void lsm3dmultiplycutofflserhslocal_(const real_t* phi, real_t* rh
                                                    const unsigned char* nb, LSMHIP_BOXP(nbb),
                                                    const unsigned char* mark_fb, const real_t* beta,
                                                    const real_t* gamma) {
+  LSMHIP_REQUIRE_ARRAYS(phi, rhs, ix, iy, iz, nb);
   const char* who = "lsm3dmultiplycutofflserhslocal_";
   const Box g = LSMHIP_BOX(g);
   const real_t c1 = *gamma - real_t(3) * *beta;
@@ -650,6 +676,7 @@ struct NormalDtOp {
 
 LSMHIP_EXPORT void lsm3dmaxnormdifflocal_(real_t* out, const real_t* f1, LSMHIP_BOXP(g1), const real_t* f2,
                                           LSMHIP_BOXP(g2), LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2, ix, iy, iz, nb);
   const char* who = "lsm3dmaxnormdifflocal_";
   double m = 0.0;
   bool any;
@@ -661,6 +688,7 @@ LSMHIP_EXPORT void lsm3dcomputestableadvectiondtlocal_(real_t* dt, const real_t*
                                                        const real_t* vz, LSMHIP_BOXP(vg), const real_t* dx,
                                                        const real_t* dy, const real_t* dz, const real_t* cfl,
                                                        LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(vx, vy, vz, ix, iy, iz, nb);
   const char* who = "lsm3dcomputestableadvectiondtlocal_";
   const Box vg = LSMHIP_BOX(vg);
   double m = -1.0;
@@ -695,6 +723,7 @@ LSMHIP_EXPORT void lsm3dcomputestablenormalveldtlocal_(real_t* dt, const real_t*
                                                        const real_t* pzm, LSMHIP_BOXP(gm), const real_t* dx,
                                                        const real_t* dy, const real_t* dz, const real_t* cfl,
                                                        LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(vn, pxp, pyp, pzp, pxm, pym, pzm, ix, iy, iz, nb);
   normal_dt_local("lsm3dcomputestablenormalveldtlocal_", dt, vn, LSMHIP_BOX(vg), real_t(0), pxp, pyp, pzp,
                   LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), *dx, *dy, *dz, *cfl, LIST, FBMASK, 1);
 }
@@ -707,6 +736,7 @@ LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldtlocal_(real_t* dt, const re
                                                             const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                             LSMHIP_BOXP(gm), const real_t* dx, const real_t* dy,
                                                             const real_t* dz, const real_t* cfl, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, ix, iy, iz, nb);
   Box none{0, -1, 0, -1, 0, -1};
   normal_dt_local("lsm3dcomputestableconstnormalveldtlocal_", dt, nullptr, none, real_t(fabs(*vn)), pxp, pyp, pzp,
                   LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), *dx, *dy, *dz, *cfl, LIST, FBMASK, 0);
@@ -732,6 +762,7 @@ struct GradNormOp {
 // reference sums serially; here the sum is FP64 compensated in a fixed order.
 LSMHIP_EXPORT void lsm3dcomputeavegradphilocal_(real_t* ave, const real_t* phi, LSMHIP_BOXP(pg), const real_t* dx,
                                                 const real_t* dy, const real_t* dz, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(phi, ix, iy, iz, nb);
   points_mean("lsm3dcomputeavegradphilocal_", LIST, FBMASK,
               GradNormOp{cv(phi, LSMHIP_BOX(pg)), real_t(0.5) / *dx, real_t(0.5) / *dy, real_t(0.5) / *dz}, ave);
 }

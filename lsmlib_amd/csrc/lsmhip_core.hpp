@@ -109,6 +109,30 @@ int take_device_flag(const char* who);
 bool check_box(const Box& outer, const Box& inner, const char* who, const char* what);
 bool check_size(const Box& b, const char* who);
 
+// Device-array arguments.  The reference's routines dereference every array
+// they are given; here a NULL array would be dereferenced by a kernel, and a
+// kernel that faults can end the whole device, not just the caller.  Every
+// exported entry point therefore tests its device-array arguments on the host
+// before it launches anything: a NULL one sets kErrArg, naming the argument,
+// and the call does nothing.  names is the comma-separated list of the
+// arguments' names (the macro's stringised argument list).
+bool report_null_array(const char* who, const char* names, int position);
+template <class... P>
+inline bool arrays_present(const char* who, const char* names, P... p) {
+  const void* const a[] = {static_cast<const void*>(p)...};
+  for (int q = 0; q < (int)sizeof...(P); q++)
+    if (a[q] == nullptr) return report_null_array(who, names, q);
+  return true;
+}
+#define LSMHIP_REQUIRE_ARRAYS(...) \
+  do {                             \
+    if (!::lsmhip::arrays_present(__func__, #__VA_ARGS__, __VA_ARGS__)) return; \
+  } while (0)
+#define LSMHIP_REQUIRE_ARRAYS_OR(rv, ...) \
+  do {                                    \
+    if (!::lsmhip::arrays_present(__func__, #__VA_ARGS__, __VA_ARGS__)) return rv; \
+  } while (0)
+
 // Launch shape for kernels that map one thread to one cell of a box: kBX x kBY
 // tiles of (i, j) in one plane k.  Workgroups are dealt to the kXcds XCDs
 // round robin in flattened-id order, and each XCD has its own L2.  The grid's x

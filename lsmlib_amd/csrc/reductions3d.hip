@@ -97,6 +97,7 @@ real_t cfl_dt(double m, real_t cfl) { return cfl / (real_t(m) + kTiny); }
 // ---------------------------------------------------------------------------
 LSMHIP_EXPORT void lsm3dmaxnormdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP(g1), const real_t* f2,
                                      LSMHIP_BOXP(g2), LSMHIP_BOXP(ib)) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2);
   const char* who = "lsm3dmaxnormdiff_";
   Box ib = LSMHIP_BOX(ib), g1 = LSMHIP_BOX(g1), g2 = LSMHIP_BOX(g2);
   *out = real_t(0);
@@ -108,6 +109,7 @@ LSMHIP_EXPORT void lsm3dmaxnormdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP(
 LSMHIP_EXPORT void lsm3dcomputestableadvectiondt_(real_t* dt, const real_t* vx, const real_t* vy, const real_t* vz,
                                                   LSMHIP_BOXP(vg), LSMHIP_BOXP(ib), const real_t* dx,
                                                   const real_t* dy, const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vx, vy, vz);
   const char* who = "lsm3dcomputestableadvectiondt_";
   Box ib = LSMHIP_BOX(ib), vg = LSMHIP_BOX(vg);
   double m = -1.0;
@@ -142,6 +144,7 @@ LSMHIP_EXPORT void lsm3dcomputestablenormalveldt_(real_t* dt, const real_t* vn, 
                                                   const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                   LSMHIP_BOXP(gm), LSMHIP_BOXP(ib), const real_t* dx,
                                                   const real_t* dy, const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vn, pxp, pyp, pzp, pxm, pym, pzm);
   normal_dt("lsm3dcomputestablenormalveldt_", dt, vn, LSMHIP_BOX(vg), real_t(0), pxp, pyp, pzp, LSMHIP_BOX(gp),
             pxm, pym, pzm, LSMHIP_BOX(gm), LSMHIP_BOX(ib), *dx, *dy, *dz, *cfl);
 }
@@ -151,6 +154,7 @@ LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldt_(real_t* dt, const real_t*
                                                        const real_t* pxm, const real_t* pym, const real_t* pzm,
                                                        LSMHIP_BOXP(gm), LSMHIP_BOXP(ib), const real_t* dx,
                                                        const real_t* dy, const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm);
   Box none{0, -1, 0, -1, 0, -1};
   normal_dt("lsm3dcomputestableconstnormalveldt_", dt, nullptr, none, real_t(fabs(*vel_n)), pxp, pyp, pzp,
             LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm), LSMHIP_BOX(ib), *dx, *dy, *dz, *cfl);
@@ -194,6 +198,7 @@ LSMHIP_EXPORT void lsm3dvolumeintegralphilessthanzero_(real_t* out, const real_t
                                                        const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
                                                        const real_t* dx, const real_t* dy, const real_t* dz,
                                                        const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi);
   bool good;
   double s = volume("lsm3dvolumeintegralphilessthanzero_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib),
                     *dx, *dy, *dz, *eps, -1, &good);
@@ -203,6 +208,7 @@ LSMHIP_EXPORT void lsm3dvolumeintegralphigreaterthanzero_(real_t* out, const rea
                                                           const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
                                                           const real_t* dx, const real_t* dy, const real_t* dz,
                                                           const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi);
   bool good;
   double s = volume("lsm3dvolumeintegralphigreaterthanzero_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg),
                     LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, +1, &good);
@@ -212,6 +218,7 @@ LSMHIP_EXPORT void lsm3dsurfaceintegral_(real_t* out, const real_t* F, LSMHIP_BO
                                          LSMHIP_BOXP(pg), const real_t* px, const real_t* py, const real_t* pz,
                                          LSMHIP_BOXP(g), LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
                                          const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi, px, py, pz);
   bool good;
   double s = surface("lsm3dsurfaceintegral_", F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g),
                      LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, &good);
@@ -220,6 +227,7 @@ LSMHIP_EXPORT void lsm3dsurfaceintegral_(real_t* out, const real_t* F, LSMHIP_BO
 LSMHIP_EXPORT void lsm3dvolumeregionphilessthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
                                                      LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
                                                      const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   bool good;
   Box none{0, -1, 0, -1, 0, -1};
   double s = volume("lsm3dvolumeregionphilessthanzero_", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), *dx,
@@ -229,6 +237,7 @@ LSMHIP_EXPORT void lsm3dvolumeregionphilessthanzero_(real_t* out, const real_t* 
 LSMHIP_EXPORT void lsm3dvolumeregionphigreaterthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
                                                         LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
                                                         const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   bool good;
   Box none{0, -1, 0, -1, 0, -1};
   double s = volume("lsm3dvolumeregionphigreaterthanzero_", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), *dx,
@@ -239,6 +248,7 @@ LSMHIP_EXPORT void lsm3dsurfaceareazerolevelset_(real_t* out, const real_t* phi,
                                                  const real_t* py, const real_t* pz, LSMHIP_BOXP(g), LSMHIP_BOXP(ib),
                                                  const real_t* dx, const real_t* dy, const real_t* dz,
                                                  const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi, px, py, pz);
   bool good;
   Box none{0, -1, 0, -1, 0, -1};
   double s = surface("lsm3dsurfaceareazerolevelset_", nullptr, none, phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g),
@@ -251,6 +261,7 @@ LSMHIP_EXPORT void lsm3dsurfaceareazerolevelset_(real_t* out, const real_t* phi,
 LSMHIP_EXPORT int lsmhip_volume_phi_lt0_f64(double* out, const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(ib),
                                             const double* dx, const double* dy, const double* dz,
                                             const double* eps) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, out, phi);
   bool good;
   Box none{0, -1, 0, -1, 0, -1};
   double s = volume("lsmhip_volume_phi_lt0_f64", nullptr, none, phi, LSMHIP_BOX(pg), LSMHIP_BOX(ib), real_t(*dx),
@@ -261,6 +272,7 @@ LSMHIP_EXPORT int lsmhip_volume_phi_lt0_f64(double* out, const real_t* phi, LSMH
 
 // Smoothed Heaviside and delta (src/toolbox/lsm_calculus_toolbox.f:25-92); host scalars.
 LSMHIP_EXPORT void lsmheaviside_(real_t* H, const real_t* x, const real_t* epsilon) {
+
   const real_t e = *epsilon, v = *x;
   if (v < -e) *H = real_t(0);
   else if (v > e) *H = real_t(1);
@@ -270,6 +282,7 @@ LSMHIP_EXPORT void lsmheaviside_(real_t* H, const real_t* x, const real_t* epsil
   }
 }
 LSMHIP_EXPORT void lsmdeltafunction_(real_t* delta, const real_t* x, const real_t* epsilon) {
+
   const real_t e = *epsilon, v = *x;
   if (fabs(v) > e) *delta = real_t(0);
   else {

@@ -87,6 +87,25 @@ bool check_box(const Box& outer, const Box& inner, const char* who, const char* 
   return false;
 }
 
+bool report_null_array(const char* who, const char* names, int position) {
+  // the position-th name of "a, b, c"
+  const char* s = names;
+  for (int q = 0; q < position && s; q++) {
+    s = strchr(s, ',');
+    if (s) s++;
+  }
+  char name[64] = "?";
+  if (s) {
+    while (*s == ' ') s++;
+    size_t n = strcspn(s, ", ");
+    if (n >= sizeof(name)) n = sizeof(name) - 1;
+    memcpy(name, s, n);
+    name[n] = 0;
+  }
+  set_error(kErrArg, "%s: array argument %s is NULL (nothing was done)", who, name);
+  return false;
+}
+
 bool check_size(const Box& b, const char* who) {
   if (b.volume() < (1LL << 31)) return true;
   set_error(kErrSize, "%s: array of %lld elements exceeds the 2^31 addressing limit", who, b.volume());

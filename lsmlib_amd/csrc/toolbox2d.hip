@@ -494,6 +494,7 @@ bool box_max2(const char* who, const Box& ib, const Op& op, double* m) {
 LSMHIP_EXPORT void lsm2dhjeno1_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real_t* pxm, real_t* pym, LSMHIP_BOXP2(gm),
                                 const real_t* phi, LSMHIP_BOXP2(pg), real_t* D1, LSMHIP_BOXP2(d1), LSMHIP_BOXP2(fb),
                                 const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1);
   (void)D1;
   hj_pm2<1>("lsm2dhjeno1_", pxp, pyp, LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm), phi, LSMHIP_BOX2(pg),
             LSMHIP_BOX2(fb), *dx, *dy);
@@ -501,6 +502,7 @@ LSMHIP_EXPORT void lsm2dhjeno1_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real
 LSMHIP_EXPORT void lsm2dhjeno2_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real_t* pxm, real_t* pym, LSMHIP_BOXP2(gm),
                                 const real_t* phi, LSMHIP_BOXP2(pg), real_t* D1, LSMHIP_BOXP2(d1), real_t* D2,
                                 LSMHIP_BOXP2(d2), LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, D2);
   (void)D1;
   (void)D2;
   hj_pm2<2>("lsm2dhjeno2_", pxp, pyp, LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm), phi, LSMHIP_BOX2(pg),
@@ -510,6 +512,7 @@ LSMHIP_EXPORT void lsm2dhjeno3_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real
                                 const real_t* phi, LSMHIP_BOXP2(pg), real_t* D1, LSMHIP_BOXP2(d1), real_t* D2,
                                 LSMHIP_BOXP2(d2), real_t* D3, LSMHIP_BOXP2(d3), LSMHIP_BOXP2(fb), const real_t* dx,
                                 const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1, D2, D3);
   (void)D1;
   (void)D2;
   (void)D3;
@@ -519,6 +522,7 @@ LSMHIP_EXPORT void lsm2dhjeno3_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real
 LSMHIP_EXPORT void lsm2dhjweno5_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), real_t* pxm, real_t* pym,
                                  LSMHIP_BOXP2(gm), const real_t* phi, LSMHIP_BOXP2(pg), real_t* D1, LSMHIP_BOXP2(d1),
                                  LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym, phi, D1);
   (void)D1;
   hj_pm2<5>("lsm2dhjweno5_", pxp, pyp, LSMHIP_BOX2(gp), pxm, pym, LSMHIP_BOX2(gm), phi, LSMHIP_BOX2(pg),
             LSMHIP_BOX2(fb), *dx, *dy);
@@ -526,6 +530,7 @@ LSMHIP_EXPORT void lsm2dhjweno5_(real_t* pxp, real_t* pyp, LSMHIP_BOXP2(gp), rea
 LSMHIP_EXPORT void lsm2dupwindhjeno1_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi, LSMHIP_BOXP2(pg),
                                       const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg), real_t* D1,
                                       LSMHIP_BOXP2(d1), LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, vx, vy, D1);
   (void)D1;
   hj_up2<1>("lsm2dupwindhjeno1_", px, py, LSMHIP_BOX2(g), phi, LSMHIP_BOX2(pg), vx, vy, LSMHIP_BOX2(vg),
             LSMHIP_BOX2(fb), *dx, *dy);
@@ -534,6 +539,7 @@ LSMHIP_EXPORT void lsm2dupwindhjeno2_(real_t* px, real_t* py, LSMHIP_BOXP2(g), c
                                       const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg), real_t* D1,
                                       LSMHIP_BOXP2(d1), real_t* D2, LSMHIP_BOXP2(d2), LSMHIP_BOXP2(fb),
                                       const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, vx, vy, D1, D2);
   (void)D1;
   (void)D2;
   hj_up2<2>("lsm2dupwindhjeno2_", px, py, LSMHIP_BOX2(g), phi, LSMHIP_BOX2(pg), vx, vy, LSMHIP_BOX2(vg),
@@ -543,6 +549,7 @@ LSMHIP_EXPORT void lsm2dupwindhjeno3_(real_t* px, real_t* py, LSMHIP_BOXP2(g), c
                                       const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg), real_t* D1,
                                       LSMHIP_BOXP2(d1), real_t* D2, LSMHIP_BOXP2(d2), real_t* D3, LSMHIP_BOXP2(d3),
                                       LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, vx, vy, D1, D2, D3);
   (void)D1;
   (void)D2;
   (void)D3;
@@ -552,6 +559,7 @@ LSMHIP_EXPORT void lsm2dupwindhjeno3_(real_t* px, real_t* py, LSMHIP_BOXP2(g), c
 LSMHIP_EXPORT void lsm2dupwindhjweno5_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi, LSMHIP_BOXP2(pg),
                                        const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg), real_t* D1,
                                        LSMHIP_BOXP2(d1), LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi, vx, vy, D1);
   (void)D1;
   hj_up2<5>("lsm2dupwindhjweno5_", px, py, LSMHIP_BOX2(g), phi, LSMHIP_BOX2(pg), vx, vy, LSMHIP_BOX2(vg),
             LSMHIP_BOX2(fb), *dx, *dy);
@@ -559,6 +567,7 @@ LSMHIP_EXPORT void lsm2dupwindhjweno5_(real_t* px, real_t* py, LSMHIP_BOXP2(g), 
 
 LSMHIP_EXPORT void lsm2dcentralgradorder2_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi,
                                            LSMHIP_BOXP2(pg), LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi);
   const char* who = "lsm2dcentralgradorder2_";
   Box fb = LSMHIP_BOX2(fb), g = LSMHIP_BOX2(g), pg = LSMHIP_BOX2(pg);
   if (fb.empty() || !inside(who, fb, {{g, "grad_phi"}}) || !cross_inside(who, fb, pg, 1, "phi")) return;
@@ -568,6 +577,7 @@ LSMHIP_EXPORT void lsm2dcentralgradorder2_(real_t* px, real_t* py, LSMHIP_BOXP2(
 }
 LSMHIP_EXPORT void lsm2dcentralgradorder4_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* phi,
                                            LSMHIP_BOXP2(pg), LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, phi);
   const char* who = "lsm2dcentralgradorder4_";
   Box fb = LSMHIP_BOX2(fb), g = LSMHIP_BOX2(g), pg = LSMHIP_BOX2(pg);
   if (fb.empty() || !inside(who, fb, {{g, "grad_phi"}}) || !cross_inside(who, fb, pg, 2, "phi")) return;
@@ -578,6 +588,7 @@ LSMHIP_EXPORT void lsm2dcentralgradorder4_(real_t* px, real_t* py, LSMHIP_BOXP2(
 }
 LSMHIP_EXPORT void lsm2dlaplacianorder2_(real_t* lap, LSMHIP_BOXP2(lg), const real_t* phi, LSMHIP_BOXP2(pg),
                                          LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(lap, phi);
   const char* who = "lsm2dlaplacianorder2_";
   Box fb = LSMHIP_BOX2(fb), lg = LSMHIP_BOX2(lg), pg = LSMHIP_BOX2(pg);
   if (fb.empty() || !inside(who, fb, {{lg, "laplacian_phi"}}) || !cross_inside(who, fb, pg, 1, "phi")) return;
@@ -588,6 +599,7 @@ LSMHIP_EXPORT void lsm2dlaplacianorder2_(real_t* lap, LSMHIP_BOXP2(lg), const re
 LSMHIP_EXPORT void lsm2daveragegradphi_(real_t* px, real_t* py, LSMHIP_BOXP2(g), const real_t* pxp,
                                         const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm, const real_t* pym,
                                         LSMHIP_BOXP2(gm), LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, pxp, pyp, pxm, pym);
   const char* who = "lsm2daveragegradphi_";
   Box fb = LSMHIP_BOX2(fb), g = LSMHIP_BOX2(g), gp = LSMHIP_BOX2(gp), gm = LSMHIP_BOX2(gm);
   if (fb.empty() || !inside(who, fb, {{g, "grad_phi"}, {gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}})) return;
@@ -597,6 +609,7 @@ LSMHIP_EXPORT void lsm2daveragegradphi_(real_t* px, real_t* py, LSMHIP_BOXP2(g),
 }
 LSMHIP_EXPORT void lsm2dgradientmagnitude_(const real_t* px, const real_t* py, real_t* mag, LSMHIP_BOXP2(g),
                                            LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(px, py, mag);
   const char* who = "lsm2dgradientmagnitude_";
   Box fb = LSMHIP_BOX2(fb), g = LSMHIP_BOX2(g);
   if (fb.empty() || !inside(who, fb, {{g, "grad_phi"}})) return;
@@ -609,6 +622,7 @@ LSMHIP_EXPORT void lsm2dgradientmagnitude_(const real_t* px, const real_t* py, r
 // Level-set RHS (src/toolbox/lsm_level_set_evolution2d.h)
 // ===========================================================================
 LSMHIP_EXPORT void lsm2dzerooutlevelseteqnrhs_(real_t* rhs, LSMHIP_BOXP2(rg)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs);
   const char* who = "lsm2dzerooutlevelseteqnrhs_";
   Box rg = LSMHIP_BOX2(rg);
   if (rg.empty() || !check_size(rg, who)) return;
@@ -618,6 +632,7 @@ LSMHIP_EXPORT void lsm2dzerooutlevelseteqnrhs_(real_t* rhs, LSMHIP_BOXP2(rg)) {
 LSMHIP_EXPORT void lsm2daddadvectiontermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg), const real_t* px, const real_t* py,
                                                   LSMHIP_BOXP2(g), const real_t* vx, const real_t* vy,
                                                   LSMHIP_BOXP2(vg), LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, vx, vy);
   const char* who = "lsm2daddadvectiontermtolserhs_";
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), g = LSMHIP_BOX2(g), vg = LSMHIP_BOX2(vg);
   if (fb.empty() || !inside(who, fb, {{rg, "lse_rhs"}, {g, "grad_phi"}, {vg, "vel"}})) return;
@@ -639,6 +654,7 @@ LSMHIP_EXPORT void lsm2daddnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg),
                                                   LSMHIP_BOXP2(gp), const real_t* pxm, const real_t* pym,
                                                   LSMHIP_BOXP2(gm), const real_t* vn, LSMHIP_BOXP2(vg),
                                                   LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, vn);
   normal2("lsm2daddnormalveltermtolserhs_", rhs, LSMHIP_BOX2(rg), pxp, pyp, LSMHIP_BOX2(gp), pxm, pym,
           LSMHIP_BOX2(gm), vn, LSMHIP_BOX2(vg), real_t(0), LSMHIP_BOX2(fb));
 }
@@ -646,6 +662,7 @@ LSMHIP_EXPORT void lsm2daddconstnormalveltermtolserhs_(real_t* rhs, LSMHIP_BOXP2
                                                        const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                        const real_t* pym, LSMHIP_BOXP2(gm), const real_t* vn,
                                                        LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym);
   if (fabs(*vn) < kZeroTol) return;  // lsm_level_set_evolution2d.f.in:231
   const Box none{0, -1, 0, -1, 0, -1};
   normal2("lsm2daddconstnormalveltermtolserhs_", rhs, LSMHIP_BOX2(rg), pxp, pyp, LSMHIP_BOX2(gp), pxm, pym,
@@ -655,6 +672,7 @@ LSMHIP_EXPORT void lsm2daddconstcurvtermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg),
                                                   LSMHIP_BOXP2(g), const real_t* pxx, const real_t* pxy,
                                                   const real_t* pyy, LSMHIP_BOXP2(g2), const real_t* b,
                                                   LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, px, py, pxx, pxy, pyy);
   const char* who = "lsm2daddconstcurvtermtolserhs_";
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), g = LSMHIP_BOX2(g), g2 = LSMHIP_BOX2(g2);
   if (fb.empty() || !inside(who, fb, {{rg, "lse_rhs"}, {g, "grad_phi"}, {g2, "grad2_phi"}})) return;
@@ -667,6 +685,7 @@ LSMHIP_EXPORT void lsm2daddconstcurvtermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg),
 LSMHIP_EXPORT void lsm2daddconstprecomputedcurvtermtolserhs_(real_t* rhs, LSMHIP_BOXP2(rg), const real_t* kappa,
                                                              LSMHIP_BOXP2(kg), const real_t* gmag, LSMHIP_BOXP2(gg),
                                                              const real_t* b, LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, kappa, gmag);
   const char* who = "lsm2daddconstprecomputedcurvtermtolserhs_";
   Box rg = LSMHIP_BOX2(rg), kg = LSMHIP_BOX2(kg), gg = LSMHIP_BOX2(gg);
   Box lb{rg.ilo, rg.ihi, *jlo_fb, rg.jhi, 0, 0};
@@ -680,6 +699,7 @@ LSMHIP_EXPORT void lsm2daddexternalandnormalveltermtolserhs_(real_t* rhs, LSMHIP
                                                              const real_t* pym, LSMHIP_BOXP2(gm), const real_t* vn,
                                                              const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg),
                                                              LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, vn, vx, vy);
   const char* who = "lsm2daddexternalandnormalveltermtolserhs_";
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), gp = LSMHIP_BOX2(gp), gm = LSMHIP_BOX2(gm), vg = LSMHIP_BOX2(vg);
   if (fb.empty() || !inside(who, fb, {{rg, "lse_rhs"}, {gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}, {vg, "vel"}}))
@@ -694,34 +714,40 @@ LSMHIP_EXPORT void lsm2daddexternalandnormalveltermtolserhs_(real_t* rhs, LSMHIP
 // ===========================================================================
 LSMHIP_EXPORT void lsm2drk1step_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs,
                                  LSMHIP_BOXP2(rg), LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk2d<1>("lsm2drk1step_", un, LSMHIP_BOX2(ng), nullptr, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), rhs, LSMHIP_BOX2(rg),
           LSMHIP_BOX2(fb), *dt);
 }
 LSMHIP_EXPORT void lsm2dtvdrk2stage1_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg),
                                       const real_t* rhs, LSMHIP_BOXP2(rg), LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk2d<1>("lsm2dtvdrk2stage1_", un, LSMHIP_BOX2(ng), nullptr, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), rhs,
           LSMHIP_BOX2(rg), LSMHIP_BOX2(fb), *dt);
 }
 LSMHIP_EXPORT void lsm2dtvdrk2stage2_(real_t* un, LSMHIP_BOXP2(ng), const real_t* u1, LSMHIP_BOXP2(g1),
                                       const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                       LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, u1, uc, rhs);
   rk2d<2>("lsm2dtvdrk2stage2_", un, LSMHIP_BOX2(ng), u1, LSMHIP_BOX2(g1), uc, LSMHIP_BOX2(cg), rhs, LSMHIP_BOX2(rg),
           LSMHIP_BOX2(fb), *dt);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage1_(real_t* un, LSMHIP_BOXP2(ng), const real_t* uc, LSMHIP_BOXP2(cg),
                                       const real_t* rhs, LSMHIP_BOXP2(rg), LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, uc, rhs);
   rk2d<1>("lsm2dtvdrk3stage1_", un, LSMHIP_BOX2(ng), nullptr, LSMHIP_BOX2(ng), uc, LSMHIP_BOX2(cg), rhs,
           LSMHIP_BOX2(rg), LSMHIP_BOX2(fb), *dt);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage2_(real_t* u2, LSMHIP_BOXP2(g2), const real_t* u1, LSMHIP_BOXP2(g1),
                                       const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                       LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(u2, u1, uc, rhs);
   rk2d<32>("lsm2dtvdrk3stage2_", u2, LSMHIP_BOX2(g2), u1, LSMHIP_BOX2(g1), uc, LSMHIP_BOX2(cg), rhs, LSMHIP_BOX2(rg),
            LSMHIP_BOX2(fb), *dt);
 }
 LSMHIP_EXPORT void lsm2dtvdrk3stage3_(real_t* un, LSMHIP_BOXP2(ng), const real_t* u2, LSMHIP_BOXP2(g2),
                                       const real_t* uc, LSMHIP_BOXP2(cg), const real_t* rhs, LSMHIP_BOXP2(rg),
                                       LSMHIP_BOXP2(fb), const real_t* dt) {
+  LSMHIP_REQUIRE_ARRAYS(un, u2, uc, rhs);
   rk2d<33>("lsm2dtvdrk3stage3_", un, LSMHIP_BOX2(ng), u2, LSMHIP_BOX2(g2), uc, LSMHIP_BOX2(cg), rhs, LSMHIP_BOX2(rg),
            LSMHIP_BOX2(fb), *dt);
 }
@@ -735,6 +761,8 @@ LSMHIP_EXPORT void lsm2dcomputereinitializationeqnrhs_(real_t* out, LSMHIP_BOXP2
                                                        const real_t* pxm, const real_t* pym, LSMHIP_BOXP2(gm),
                                                        LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy,
                                                        const int* use_phi0_for_sgn) {
+  LSMHIP_REQUIRE_ARRAYS(out, phi, pxp, pyp, pxm, pym);
+  if (*use_phi0_for_sgn == 1) LSMHIP_REQUIRE_ARRAYS(phi0);
   const char* who = "lsm2dcomputereinitializationeqnrhs_";
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), pg = LSMHIP_BOX2(pg), p0g = LSMHIP_BOX2(p0g), gp = LSMHIP_BOX2(gp),
       gm = LSMHIP_BOX2(gm);
@@ -754,6 +782,7 @@ LSMHIP_EXPORT void lsm2dcomputeorthogonalizationeqnrhs_(real_t* rhs, LSMHIP_BOXP
                                                         LSMHIP_BOXP2(qp), const real_t* qxm, const real_t* qym,
                                                         LSMHIP_BOXP2(qm), LSMHIP_BOXP2(fb), const real_t* dx,
                                                         const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pxm, pym, psi, qxp, qyp, qxm, qym);
   const char* who = "lsm2dcomputeorthogonalizationeqnrhs_";
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), gp = LSMHIP_BOX2(gp), gm = LSMHIP_BOX2(gm), sg = LSMHIP_BOX2(sg),
       qp = LSMHIP_BOX2(qp), qm = LSMHIP_BOX2(qm);
@@ -779,6 +808,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder2_(real_t* kappa, LSMHIP_BOXP2(
                                                     LSMHIP_BOXP2(pg), const real_t* px, const real_t* py,
                                                     const real_t* gmag, LSMHIP_BOXP2(gg), LSMHIP_BOXP2(fb),
                                                     const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, gmag);
   const char* who = "lsm2dcomputemeancurvatureorder2_";
   Box fb = LSMHIP_BOX2(fb), kg = LSMHIP_BOX2(kg), pg = LSMHIP_BOX2(pg), gg = LSMHIP_BOX2(gg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}, {gg, "grad_phi"}}) ||
@@ -793,6 +823,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder4_(real_t* kappa, LSMHIP_BOXP2(
                                                     LSMHIP_BOXP2(pg), const real_t* px, const real_t* py,
                                                     const real_t* gmag, LSMHIP_BOXP2(gg), LSMHIP_BOXP2(fb),
                                                     const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, gmag);
   const char* who = "lsm2dcomputemeancurvatureorder4_";
   Box fb = LSMHIP_BOX2(fb), kg = LSMHIP_BOX2(kg), pg = LSMHIP_BOX2(pg), gg = LSMHIP_BOX2(gg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}}) || !check_box(gg, fb.widen(0, 2, 0), who, "grad_phi") ||
@@ -807,6 +838,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvatureorder4_(real_t* kappa, LSMHIP_BOXP2(
 LSMHIP_EXPORT void lsm2dcomputemeancurvature_(real_t* kappa, LSMHIP_BOXP2(kg), const real_t* px, const real_t* py,
                                               LSMHIP_BOXP2(gg), const real_t* pxx, const real_t* pxy,
                                               const real_t* pyy, LSMHIP_BOXP2(hg), LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, px, py, pxx, pxy, pyy);
   const char* who = "lsm2dcomputemeancurvature_";
   Box fb = LSMHIP_BOX2(fb), kg = LSMHIP_BOX2(kg), gg = LSMHIP_BOX2(gg), hg = LSMHIP_BOX2(hg);
   if (fb.empty() || !inside(who, fb, {{kg, "kappa"}, {gg, "grad_phi"}, {hg, "hessian_phi"}})) return;
@@ -816,6 +848,7 @@ LSMHIP_EXPORT void lsm2dcomputemeancurvature_(real_t* kappa, LSMHIP_BOXP2(kg), c
 }
 LSMHIP_EXPORT void lsm2dcomputeunitnormal_(real_t* nx, real_t* ny, LSMHIP_BOXP2(ng), const real_t* px,
                                            const real_t* py, LSMHIP_BOXP2(gg), LSMHIP_BOXP2(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, px, py);
   const char* who = "lsm2dcomputeunitnormal_";
   Box fb = LSMHIP_BOX2(fb), ng = LSMHIP_BOX2(ng), gg = LSMHIP_BOX2(gg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}})) return;
@@ -827,6 +860,7 @@ LSMHIP_EXPORT void lsm2dcomputesignedunitnormal_(real_t* nx, real_t* ny, LSMHIP_
                                                  const real_t* py, LSMHIP_BOXP2(gg), const real_t* phi,
                                                  LSMHIP_BOXP2(pg), LSMHIP_BOXP2(fb), const real_t* dx,
                                                  const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(nx, ny, px, py, phi);
   const char* who = "lsm2dcomputesignedunitnormal_";
   Box fb = LSMHIP_BOX2(fb), ng = LSMHIP_BOX2(ng), gg = LSMHIP_BOX2(gg), pg = LSMHIP_BOX2(pg);
   if (fb.empty() || !inside(who, fb, {{ng, "normal"}, {gg, "grad_phi"}, {pg, "phi"}})) return;
@@ -840,6 +874,7 @@ LSMHIP_EXPORT void lsm2dcomputefieldextensioneqnrhs_(real_t* rhs, LSMHIP_BOXP2(r
                                                      const real_t* sx, const real_t* sy, LSMHIP_BOXP2(ug),
                                                      const real_t* nx, const real_t* ny, LSMHIP_BOXP2(ng),
                                                      LSMHIP_BOXP2(fb), const real_t* dx, const real_t* dy) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, S, phi, sx, sy, nx, ny);
   const char* who = "lsm2dcomputefieldextensioneqnrhs_";
   (void)S;  // part of the reference signature; the RHS reads only its upwind gradient
   Box fb = LSMHIP_BOX2(fb), rg = LSMHIP_BOX2(rg), pg = LSMHIP_BOX2(pg), ug = LSMHIP_BOX2(ug), ng = LSMHIP_BOX2(ng);
@@ -857,6 +892,7 @@ LSMHIP_EXPORT void lsm2dcomputefieldextensioneqnrhs_(real_t* rhs, LSMHIP_BOXP2(r
 // ===========================================================================
 LSMHIP_EXPORT void lsm2dmaxnormdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP2(g1), const real_t* f2,
                                      LSMHIP_BOXP2(g2), LSMHIP_BOXP2(ib)) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2);
   const char* who = "lsm2dmaxnormdiff_";
   Box ib = LSMHIP_BOX2(ib), g1 = LSMHIP_BOX2(g1), g2 = LSMHIP_BOX2(g2);
   *out = real_t(0);
@@ -867,6 +903,7 @@ LSMHIP_EXPORT void lsm2dmaxnormdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP2
 LSMHIP_EXPORT void lsm2dcomputestableadvectiondt_(real_t* dt, const real_t* vx, const real_t* vy, LSMHIP_BOXP2(vg),
                                                   LSMHIP_BOXP2(ib), const real_t* dx, const real_t* dy,
                                                   const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vx, vy);
   const char* who = "lsm2dcomputestableadvectiondt_";
   Box ib = LSMHIP_BOX2(ib), vg = LSMHIP_BOX2(vg);
   double m = -1.0;
@@ -895,6 +932,7 @@ LSMHIP_EXPORT void lsm2dcomputestablenormalveldt_(real_t* dt, const real_t* vn, 
                                                   const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                   const real_t* pym, LSMHIP_BOXP2(gm), LSMHIP_BOXP2(ib),
                                                   const real_t* dx, const real_t* dy, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vn, pxp, pyp, pxm, pym);
   normal_dt2("lsm2dcomputestablenormalveldt_", dt, vn, LSMHIP_BOX2(vg), real_t(0), pxp, pyp, LSMHIP_BOX2(gp), pxm, pym,
              LSMHIP_BOX2(gm), LSMHIP_BOX2(ib), *dx, *dy, *cfl);
 }
@@ -902,6 +940,7 @@ LSMHIP_EXPORT void lsm2dcomputestableconstnormalveldt_(real_t* dt, const real_t*
                                                        const real_t* pyp, LSMHIP_BOXP2(gp), const real_t* pxm,
                                                        const real_t* pym, LSMHIP_BOXP2(gm), LSMHIP_BOXP2(ib),
                                                        const real_t* dx, const real_t* dy, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pxm, pym);
   const Box none{0, -1, 0, -1, 0, -1};
   normal_dt2("lsm2dcomputestableconstnormalveldt_", dt, nullptr, none, real_t(fabs(*vn)), pxp, pyp, LSMHIP_BOX2(gp),
              pxm, pym, LSMHIP_BOX2(gm), LSMHIP_BOX2(ib), *dx, *dy, *cfl);
@@ -936,23 +975,27 @@ static void surface2(const char* who, real_t* out, const real_t* F, const Box& f
 LSMHIP_EXPORT void lsm2dvolumeintegralphilessthanzero_(real_t* out, const real_t* F, LSMHIP_BOXP2(fg),
                                                        const real_t* phi, LSMHIP_BOXP2(pg), LSMHIP_BOXP2(ib),
                                                        const real_t* dx, const real_t* dy, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi);
   region2("lsm2dvolumeintegralphilessthanzero_", out, F, LSMHIP_BOX2(fg), phi, LSMHIP_BOX2(pg), LSMHIP_BOX2(ib), *dx,
           *dy, *eps, -1);
 }
 LSMHIP_EXPORT void lsm2dvolumeintegralphigreaterthanzero_(real_t* out, const real_t* F, LSMHIP_BOXP2(fg),
                                                           const real_t* phi, LSMHIP_BOXP2(pg), LSMHIP_BOXP2(ib),
                                                           const real_t* dx, const real_t* dy, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi);
   region2("lsm2dvolumeintegralphigreaterthanzero_", out, F, LSMHIP_BOX2(fg), phi, LSMHIP_BOX2(pg), LSMHIP_BOX2(ib),
           *dx, *dy, *eps, 1);
 }
 LSMHIP_EXPORT void lsm2dsurfaceintegral_(real_t* out, const real_t* F, LSMHIP_BOXP2(fg), const real_t* phi,
                                          LSMHIP_BOXP2(pg), const real_t* px, const real_t* py, LSMHIP_BOXP2(gg),
                                          LSMHIP_BOXP2(ib), const real_t* dx, const real_t* dy, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi, px, py);
   surface2("lsm2dsurfaceintegral_", out, F, LSMHIP_BOX2(fg), phi, LSMHIP_BOX2(pg), px, py, LSMHIP_BOX2(gg),
            LSMHIP_BOX2(ib), *dx, *dy, *eps);
 }
 LSMHIP_EXPORT void lsm2darearegionphilessthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP2(pg), LSMHIP_BOXP2(ib),
                                                    const real_t* dx, const real_t* dy, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   const Box none{0, -1, 0, -1, 0, -1};
   region2("lsm2darearegionphilessthanzero_", out, nullptr, none, phi, LSMHIP_BOX2(pg), LSMHIP_BOX2(ib), *dx, *dy,
           *eps, -1);
@@ -960,6 +1003,7 @@ LSMHIP_EXPORT void lsm2darearegionphilessthanzero_(real_t* out, const real_t* ph
 LSMHIP_EXPORT void lsm2darearegionphigreaterthanzero_(real_t* out, const real_t* phi, LSMHIP_BOXP2(pg),
                                                       LSMHIP_BOXP2(ib), const real_t* dx, const real_t* dy,
                                                       const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   const Box none{0, -1, 0, -1, 0, -1};
   region2("lsm2darearegionphigreaterthanzero_", out, nullptr, none, phi, LSMHIP_BOX2(pg), LSMHIP_BOX2(ib), *dx, *dy,
           *eps, 1);
@@ -967,6 +1011,7 @@ LSMHIP_EXPORT void lsm2darearegionphigreaterthanzero_(real_t* out, const real_t*
 LSMHIP_EXPORT void lsm2dperimeterzerolevelset_(real_t* out, const real_t* phi, LSMHIP_BOXP2(pg), const real_t* px,
                                                const real_t* py, LSMHIP_BOXP2(gg), LSMHIP_BOXP2(ib), const real_t* dx,
                                                const real_t* dy, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi, px, py);
   const Box none{0, -1, 0, -1, 0, -1};
   surface2("lsm2dperimeterzerolevelset_", out, nullptr, none, phi, LSMHIP_BOX2(pg), px, py, LSMHIP_BOX2(gg),
            LSMHIP_BOX2(ib), *dx, *dy, *eps);
@@ -976,24 +1021,31 @@ LSMHIP_EXPORT void lsm2dperimeterzerolevelset_(real_t* out, const real_t* phi, L
 // Ghost fills (src/boundary_conditions/lsm_boundary_conditions2d.h)
 // ===========================================================================
 LSMHIP_EXPORT void lsm2dlinearextrapolation_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dlinearextrapolation_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kLinear);
 }
 LSMHIP_EXPORT void lsm2dsignedlinearextrapolation_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dsignedlinearextrapolation_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kSigned);
 }
 LSMHIP_EXPORT void lsm2dcopyextrapolation_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dcopyextrapolation_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kCopy);
 }
 // The four homogeneous Neumann fills are copy extrapolation (lsm_boundary_conditions2d.f:360-510).
 LSMHIP_EXPORT void lsm2dhomogeneousneumanneno1_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dhomogeneousneumanneno1_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm2dhomogeneousneumanneno2_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dhomogeneousneumanneno2_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm2dhomogeneousneumanneno3_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dhomogeneousneumanneno3_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kCopy);
 }
 LSMHIP_EXPORT void lsm2dhomogeneousneumannweno5_(real_t* phi, LSMHIP_BOXP2(gb), LSMHIP_BOXP2(fb), const int* idx) {
+  LSMHIP_REQUIRE_ARRAYS(phi);
   face2("lsm2dhomogeneousneumannweno5_", phi, LSMHIP_BOX2(gb), LSMHIP_BOX2(fb), *idx, kCopy);
 }

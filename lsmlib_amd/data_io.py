@@ -17,8 +17,14 @@ from .lib import LsmHipError, load
 ACCURACY = {"eno1": 0, "eno2": 1, "eno3": 2, "weno5": 3}
 
 
+_GRID_TYPES = {}
+
+
 def grid_struct(real):
-    """ctypes mirror of LsmhipGrid for LSMLIB_REAL = real (c_double or c_float)."""
+    """ctypes mirror of LsmhipGrid (the reference's Grid) for LSMLIB_REAL = real
+    (c_double or c_float); one class per real type."""
+    if real in _GRID_TYPES:
+        return _GRID_TYPES[real]
     fields = [("num_dims", ctypes.c_int)]
     fields += [(k, real * 3) for k in ("x_lo", "x_hi", "x_lo_ghostbox", "x_hi_ghostbox")]
     fields += [("grid_dims", ctypes.c_int * 3), ("grid_dims_ghostbox", ctypes.c_int * 3), ("dx", real * 3),
@@ -28,7 +34,8 @@ def grid_struct(real):
     fields += [("num_nb_levels", ctypes.c_int)]
     fields += [(k, ctypes.c_ubyte) for k in ("mark_gb", "mark_D1", "mark_D2", "mark_D3", "mark_fb")]
     fields += [("beta", real), ("gamma", real)]
-    return type("LsmhipGrid_%s" % real.__name__, (ctypes.Structure,), {"_fields_": fields})
+    _GRID_TYPES[real] = type("LsmhipGrid_%s" % real.__name__, (ctypes.Structure,), {"_fields_": fields})
+    return _GRID_TYPES[real]
 
 
 _STRUCTS = {}

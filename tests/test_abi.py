@@ -26,6 +26,24 @@ def declared_symbols():
     return syms
 
 
+# the reference's C utilities, exported under their own names
+REFERENCE_C = ["linearExtrapolationBC", "signedLinearExtrapolationBC", "copyExtrapolationBC", "homogeneousNeumannBC",
+               "createGridSetDx", "createGridSetDxDyDz", "createGridSetGridDims", "destroyGrid",
+               "writeGridToBinaryFile", "readGridFromBinaryFile", "writeDataArray", "writeDataArrayNoGrid",
+               "readDataArray"]
+
+
+def test_reference_named_utilities_declared_and_exported():
+    txt = " ".join(open(h).read() for h in glob.glob(os.path.join(ROOT, "include", "*.h")))
+    for name in REFERENCE_C:
+        assert re.search(r"\b%s\(" % name, txt), name
+    import torch  # noqa: F401
+    for path in LIBS:
+        if os.path.exists(path):
+            lib = ctypes.CDLL(path)
+            assert all(hasattr(lib, n) for n in REFERENCE_C), path
+
+
 def test_headers_declare_the_spec_table():
     syms = declared_symbols()
     missing = [s for s in capi.SPECS if s not in syms]
