@@ -237,6 +237,25 @@ void oracle_lsm3dcomputeavegradphilocal(double *ave, const double *phi, BOXP(pg)
   *ave = s;
 }
 
+/* The addends of that mean's sum, one per admitted band point in list order
+ * (*count of them). */
+void oracle_avegradphilocal_terms(double *terms, int *count, const double *phi, BOXP(pg), const double *dx,
+                                  const double *dy, const double *dz, const int *ix, const int *iy,
+                                  const int *iz, const int *nlo, const int *nhi, const unsigned char *nb,
+                                  BOXP(nbg), const unsigned char *mark_fb) {
+  orc_arr F = VIEW(phi, pg);
+  orc_mask N = MVIEW(nb, nbg);
+  double fx = 0.5 / *dx, fy = 0.5 / *dy, fz = 0.5 / *dz;
+  int c = 0;
+  FOR_BAND(*nlo, *nhi, N, *mark_fb) {
+    double x = (AT(F, i + 1, j, k) - AT(F, i - 1, j, k)) * fx;
+    double y = (AT(F, i, j + 1, k) - AT(F, i, j - 1, k)) * fy;
+    double z = (AT(F, i, j, k + 1) - AT(F, i, j, k - 1)) * fz;
+    terms[c++] = sqrt(x * x + y * y + z * z);
+  }
+  *count = c;
+}
+
 /* lsm3dGradientMagnitudeLocal, :1186-1282 (uses zero_tol, unlike the whole-grid 1e-8) */
 void oracle_lsm3dgradientmagnitudelocal(const double *px, const double *py, const double *pz, double *mag,
                                         BOXP(g), const int *ix, const int *iy, const int *iz, const int *nlo,

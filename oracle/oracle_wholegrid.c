@@ -814,6 +814,28 @@ void oracle_lsm3dsurfaceintegral(double *intF, const double *F, BOXP(fg), const 
   *intF = s;
 }
 
+/* Per-cell addends of the smoothed volume and area sums below, in the
+ * reference's operations: each sum adds exactly these values in loop order,
+ * so the *_terms exports (one addend per cell of ib, 0 where the reference
+ * adds nothing) let a test form the exactly rounded sum of the same terms. */
+static inline double vol_lt0_term(double p, double e, double dV) {
+  double q = p / e;
+  if (p < -e) return dV;
+  if (p < e) return 0.5 * (1.0 - q - ORC_ONE_OVER_PI * sin(ORC_PI * q)) * dV;
+  return 0.0;
+}
+static inline double vol_gt0_term(double p, double e, double dV) {
+  double q = p / e;
+  if (p > e) return dV;
+  if (p > -e) return 0.5 * (1.0 + q + ORC_ONE_OVER_PI * sin(ORC_PI * q)) * dV;
+  return 0.0;
+}
+static inline double area_term(double p, double x, double y, double z, double e, double ie, double dV) {
+  if (!(fabs(p) < e)) return 0.0;
+  double delta = 0.5 * ie * (1.0 + cos(ORC_PI * p * ie));
+  return delta * sqrt(x * x + y * y + z * z) * dV;
+}
+
 /* lsm_geometry3d_fort.f.in:395-465 */
 void oracle_lsm3dvolumeregionphilessthanzero(double *vol, const double *phi, BOXP(pg), BOXP(ib),
                                              const double *dx, const double *dy, const double *dz,
@@ -822,9 +844,8 @@ void oracle_lsm3dvolumeregionphilessthanzero(double *vol, const double *phi, BOX
   double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
   obox b = BOX(ib);
   FOR_FB(b) {
-    double p = AT(P, i, j, k), q = p / e;
-    if (p < -e) s = s + dV;
-    else if (p < e) s = s + 0.5 * (1.0 - q - ORC_ONE_OVER_PI * sin(ORC_PI * q)) * dV;
+    double p = AT(P, i, j, k);
+    if (p < e) s = s + vol_lt0_term(p, e, dV);
   }
   *vol = s;
 }
@@ -837,9 +858,8 @@ void oracle_lsm3dvolumeregionphigreaterthanzero(double *vol, const double *phi, 
   double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
   obox b = BOX(ib);
   FOR_FB(b) {
-    double p = AT(P, i, j, k), q = p / e;
-    if (p > e) s = s + dV;
-    else if (p > -e) s = s + 0.5 * (1.0 + q + ORC_ONE_OVER_PI * sin(ORC_PI * q)) * dV;
+    double p = AT(P, i, j, k);
+    if (p > -e) s = s + vol_gt0_term(p, e, dV);
   }
   *vol = s;
 }
@@ -854,13 +874,31 @@ void oracle_lsm3dsurfaceareazerolevelset(double *area, const double *phi, BOXP(p
   obox b = BOX(ib);
   FOR_FB(b) {
     double p = AT(P, i, j, k);
-    if (fabs(p) < e) {
-      double delta = 0.5 * ie * (1.0 + cos(ORC_PI * p * ie));
-      double x = AT(X, i, j, k), y = AT(Y, i, j, k), z = AT(Z, i, j, k);
-      s = s + delta * sqrt(x * x + y * y + z * z) * dV;
-    }
+    if (fabs(p) < e) s = s + area_term(p, AT(X, i, j, k), AT(Y, i, j, k), AT(Z, i, j, k), e, ie, dV);
   }
   *area = s;
+}
+
+/* The addends of the three sums above, one per cell of ib in (k, j, i) loop
+ * order (which: 0 volume of phi < 0, 1 volume of phi > 0, 2 area; px, py, pz
+ * are read for the area only). */
+void oracle_integral_terms(double *terms, int which, const double *phi, BOXP(pg), const double *px,
+                           const double *py, const double *pz, BOXP(gg), BOXP(ib), const double *dx,
+                           const double *dy, const double *dz, const double *epsilon) {
+  orc_arr P = VIEW(phi, pg);
+  double dV = *dx * *dy * *dz, e = *epsilon, ie = 1.0 / e;
+  obox b = BOX(ib);
+  long t = 0;
+  FOR_FB(b) {
+    double p = AT(P, i, j, k);
+    if (which == 0) terms[t] = vol_lt0_term(p, e, dV);
+    else if (which == 1) terms[t] = vol_gt0_term(p, e, dV);
+    else {
+      orc_arr X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg);
+      terms[t] = area_term(p, AT(X, i, j, k), AT(Y, i, j, k), AT(Z, i, j, k), e, ie, dV);
+    }
+    t++;
+  }
 }
 
 /* ---------------------------------------------------------------------------

@@ -132,8 +132,10 @@ def run(impl, sym, case, device=None, skip_arrays=(), real32=False):
     return res, out
 
 
-def assert_same(a, b, sym, tol=0.0, skip=()):
-    """Bitwise (tol == 0) or max-abs comparison of two run() results."""
+def assert_same(a, b, sym, tol=0.0, skip=(), out_abs=None):
+    """Bitwise (tol == 0) or max-abs comparison of two run() results.
+    out_abs: compare the scalar outputs to that absolute tolerance instead (the
+    arrays still by tol)."""
     arrs_a, out_a = a
     arrs_b, out_b = b
     for name in arrs_a:
@@ -150,7 +152,12 @@ def assert_same(a, b, sym, tol=0.0, skip=()):
         else:
             d = np.max(np.abs(x.astype(np.float64) - y.astype(np.float64))) if x.size else 0.0
             assert d <= tol, "%s: array %s max-abs diff %g > %g" % (sym, name, d, tol)
-    if tol == 0.0:
+    if out_abs is not None:
+        oa = out_a if isinstance(out_a, tuple) else (out_a,)
+        ob = out_b if isinstance(out_b, tuple) else (out_b,)
+        for p, q in zip(oa, ob):
+            assert abs(p - q) <= out_abs, "%s: output %r vs %r (|diff| %.3g > %.3g)" % (sym, p, q, abs(p - q), out_abs)
+    elif tol == 0.0:
         assert out_a == out_b, "%s: outputs differ %r vs %r" % (sym, out_a, out_b)
     elif out_a is not None:
         oa = out_a if isinstance(out_a, tuple) else (out_a,)

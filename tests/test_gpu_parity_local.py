@@ -34,7 +34,7 @@ def test_local_bitwise(hip, sym):
     a = H.run(H.oracle(), sym, c)
     b = H.run(hip.api, sym, c, device="cuda")
     hip.check()
-    H.assert_same(a, b, sym, tol=1e-12 if sym in SUMS else 0.0)
+    H.assert_same(a, b, sym, out_abs=1e-13 if sym in SUMS else None)
 
 
 @pytest.mark.parametrize("level", [1, 3, 4])
