@@ -7,6 +7,11 @@ form of HJ_WENO5 -> ADD_NORMAL_VEL_TERM -> TVD_RK3_STAGEn of the reference),
 periodic ghost cells, fixed dt.  Reverse: lsmhip_weno5_stage_normal_adjoint per
 stage and lsmhip_periodic_fill_adjoint per ghost fill, with the forward
 recomputed from checkpoints stored every `checkpoint_every` steps.
+
+Each forward run returns its own Trajectory (checkpoints, speed, step), and
+the reverse sweep takes the trajectory it walks back, so that several forward
+runs on one engine (several autograd nodes before one backward) keep their
+own states.
 """
 from __future__ import annotations
 
@@ -17,6 +22,16 @@ import numpy as np
 
 from .lib import LsmHipError
 from .solver import Grid, Solver, SplitMix64, phi_sphere
+
+
+class Trajectory:
+    """What the reverse sweep of one forward run needs: the checkpoints (phi at
+    steps 0, k, 2k, ...), the speed field and the step."""
+
+    __slots__ = ("checkpoints", "V", "dt", "steps")
+
+    def __init__(self, checkpoints, V, dt, steps):
+        self.checkpoints, self.V, self.dt, self.steps = checkpoints, V, dt, steps
 
 
 class NormalMotionAdjoint:
@@ -34,8 +49,7 @@ class NormalMotionAdjoint:
                          [P, P, P, P, P] + box + box + [I, R, P, R, R, R, R, P])
         self._fold = L.fn("lsmhip_periodic_fill_adjoint", [P] + box + box)
         self._scratch = None
-        self.checkpoints = []
-        self.V = None
+        self.last = None  # the Trajectory of the latest forward run with checkpoints
 
     # -- forward -------------------------------------------------------------------
     def _step(self, phi, p1, p2, V):
@@ -47,29 +61,39 @@ class NormalMotionAdjoint:
         s.stage_normal(phi, p2, phi, 3, dt, vn=0.0, vn_field=V)
         s.fill_ghosts(phi)
 
-    def forward(self, phi0, V, keep_checkpoints=True):
+    def run(self, phi0, V, keep_checkpoints=True):
         """phi0, V: tensors on the ghost box (ghost entries of phi0 are
-        replaced by the periodic fill; V is read on the fill box)."""
+        replaced by the periodic fill; V is read on the fill box).  Returns
+        (phi_T, Trajectory); the trajectory holds no checkpoints when
+        keep_checkpoints is false."""
         s = self.solver
         phi = phi0.detach().to(s.dtype).clone().contiguous()
-        self.V = V.detach().to(s.dtype).contiguous()
+        Vs = V.detach().to(s.dtype).contiguous().clone()
         s.fill_ghosts(phi)
         p1, p2 = s.field(), s.field()
-        self.checkpoints = []
+        checkpoints = []
         for n in range(self.steps):
             if keep_checkpoints and n % self.k == 0:
-                self.checkpoints.append(phi.clone())
-            self._step(phi, p1, p2, self.V)
+                checkpoints.append(phi.clone())
+            self._step(phi, p1, p2, Vs)
+        return phi, Trajectory(checkpoints, Vs, self.dt, self.steps)
+
+    def forward(self, phi0, V, keep_checkpoints=True):
+        """phi_T of run(); with checkpoints, its trajectory becomes the one
+        backward() walks back when it is given none."""
+        phi, traj = self.run(phi0, V, keep_checkpoints)
+        if keep_checkpoints:
+            self.last = traj
         return phi
 
     # -- reverse ----------------------------------------------------------------------
-    def _stage_adjoint(self, lam_us, lam_uc, lam_v, lam_out, us, stage):
+    def _stage_adjoint(self, lam_us, lam_uc, lam_v, lam_out, us, stage, V, dt):
         g, s = self.grid, self.solver
         if self._scratch is None:
             self._scratch = self.torch.empty((21,) + g.shape, dtype=s.dtype, device=s.device)
         rc = self._adj(lam_us.data_ptr(), lam_uc.data_ptr() if lam_uc is not None else None,
                        lam_v.data_ptr() if lam_v is not None else None, lam_out.data_ptr(), us.data_ptr(),
-                       *s._gb(), *s._fb(), stage, 0.0, self.V.data_ptr(), self.dt, *g.dx, self._scratch.data_ptr())
+                       *s._gb(), *s._fb(), stage, 0.0, V.data_ptr(), dt, *g.dx, self._scratch.data_ptr())
         if rc != 0:
             s.lib.check()
             raise LsmHipError("stage adjoint failed")
@@ -80,40 +104,44 @@ class NormalMotionAdjoint:
             s.lib.check()
             raise LsmHipError("periodic fill adjoint failed")
 
-    def backward(self, lam_T):
-        """Given dJ/dphi_T (ghost box), returns (dJ/dphi0, dJ/dV) on the ghost box."""
+    def backward(self, lam_T, trajectory=None):
+        """Given dJ/dphi_T (ghost box) of the run that produced `trajectory`
+        (default: the latest forward run with checkpoints), returns
+        (dJ/dphi0, dJ/dV) on the ghost box."""
         s, t = self.solver, self.torch
-        if not self.checkpoints:
+        tr = trajectory if trajectory is not None else self.last
+        if tr is None or not tr.checkpoints:
             raise LsmHipError("backward needs a forward run with checkpoints")
+        V, dt, steps = tr.V, tr.dt, tr.steps
         lam = lam_T.detach().to(s.dtype).clone().contiguous()
         lam_v = s.field()
-        nseg = len(self.checkpoints)
+        nseg = len(tr.checkpoints)
         states = None
         for seg in range(nseg - 1, -1, -1):
             n0 = seg * self.k
-            n1 = min(n0 + self.k, self.steps)
+            n1 = min(n0 + self.k, steps)
             # recompute the segment, keeping the three stage inputs of every step
             if states is None:
                 states = [[s.field() for _ in range(3)] for _ in range(self.k)]
-            phi = self.checkpoints[seg].clone()
+            phi = tr.checkpoints[seg].clone()
             for m in range(n1 - n0):
                 u0, u1, u2 = states[m]
                 u0.copy_(phi)
-                s.stage_normal(u1, u0, None, 1, self.dt, vn=0.0, vn_field=self.V)
+                s.stage_normal(u1, u0, None, 1, dt, vn=0.0, vn_field=V)
                 s.fill_ghosts(u1)
-                s.stage_normal(u2, u1, u0, 2, self.dt, vn=0.0, vn_field=self.V)
+                s.stage_normal(u2, u1, u0, 2, dt, vn=0.0, vn_field=V)
                 s.fill_ghosts(u2)
-                s.stage_normal(phi, u2, u0, 3, self.dt, vn=0.0, vn_field=self.V)
+                s.stage_normal(phi, u2, u0, 3, dt, vn=0.0, vn_field=V)
                 s.fill_ghosts(phi)
             for m in range(n1 - n0 - 1, -1, -1):
                 u0, u1, u2 = states[m]
                 self._fold_ghosts(lam)  # adjoint of fill(phi_{n+1})
                 lam0, lam1, lam2 = s.field(), s.field(), s.field()
-                self._stage_adjoint(lam2, lam0, lam_v, lam, u2, 3)
+                self._stage_adjoint(lam2, lam0, lam_v, lam, u2, 3, V, dt)
                 self._fold_ghosts(lam2)
-                self._stage_adjoint(lam1, lam0, lam_v, lam2, u1, 2)
+                self._stage_adjoint(lam1, lam0, lam_v, lam2, u1, 2, V, dt)
                 self._fold_ghosts(lam1)
-                self._stage_adjoint(lam0, None, lam_v, lam1, u0, 1)
+                self._stage_adjoint(lam0, None, lam_v, lam1, u0, 1, V, dt)
                 lam = lam0
         self._fold_ghosts(lam)  # adjoint of the initial fill
         t.cuda.synchronize()
@@ -137,13 +165,15 @@ def _autograd_classes():
 
         @staticmethod
         def forward(ctx, phi0, V, engine):
+            # the node keeps its own trajectory: later runs on the same engine do not touch it
+            out, ctx.trajectory = engine.run(phi0, V)
             ctx.engine = engine
-            out = engine.forward(phi0, V)
             return out.to(phi0.dtype)
 
         @staticmethod
         def backward(ctx, grad_out):
-            lam0, lam_v = ctx.engine.backward(grad_out.contiguous())
+            lam0, lam_v = ctx.engine.backward(grad_out.contiguous(), ctx.trajectory)
+            ctx.trajectory = None
             return lam0.to(grad_out.dtype), lam_v.to(grad_out.dtype), None
 
     class Mismatch(torch.autograd.Function):

@@ -77,9 +77,7 @@ def test_stage_adjoint_matches_torch_autograd(stage):
     d_us, d_uc, d_V, d_lam = ghost(us), ghost(uc), ghost(V), ghost(lam_out)
     s.fill_ghosts(d_us)
     lam_us, lam_uc, lam_v = s.field(), s.field(), s.field()
-    eng.V = d_V
-    eng.dt = dt
-    eng._stage_adjoint(lam_us, lam_uc, lam_v, d_lam, d_us, stage)
+    eng._stage_adjoint(lam_us, lam_uc, lam_v, d_lam, d_us, stage, d_V, dt)
     eng._fold_ghosts(lam_us)
     s.lib.check()
 
@@ -95,6 +93,34 @@ def test_stage_adjoint_matches_torch_autograd(stage):
     assert rel(g.interior(s.host(lam_v)), gV.numpy()) < 1e-10
     if stage != 1:
         assert rel(g.interior(s.host(lam_uc)), guc.numpy()) < 1e-12
+
+
+def test_two_forward_runs_on_one_engine_keep_their_own_gradients():
+    """Two Evolve nodes on one engine before a single backward (a summed loss)
+    give, for each speed and phi0, the gradients of separate runs: each node
+    keeps its own checkpoints and speed.  A run without checkpoints in between
+    (ShapeMatching.value) does not disturb them."""
+    import torch
+    from lsmlib_amd.adjoint import ShapeMatching, autograd_functions
+    P = ShapeMatching(n=32, precision="f64", steps=12, checkpoint_every=4)
+    Evolve, Mismatch = autograd_functions()
+    a1 = P.a.detach().clone()
+    a2 = (P.a * 1.7 + 0.05).detach().clone()
+    want = []
+    for a in (a1, a2):
+        ai = a.clone().requires_grad_(True)
+        pi = P.phi0.detach().clone().requires_grad_(True)
+        L = Mismatch.apply(Evolve.apply(pi, P.speed(ai), P.engine), P.target, P.engine.solver, P.eps)
+        want.append(torch.autograd.grad(L, (ai, pi)))
+    b1, b2 = a1.clone().requires_grad_(True), a2.clone().requires_grad_(True)
+    p1, p2 = (P.phi0.detach().clone().requires_grad_(True) for _ in range(2))
+    L1 = Mismatch.apply(Evolve.apply(p1, P.speed(b1), P.engine), P.target, P.engine.solver, P.eps)
+    L2 = Mismatch.apply(Evolve.apply(p2, P.speed(b2), P.engine), P.target, P.engine.solver, P.eps)
+    P.value(a1, P.phi0)  # a forward run without checkpoints between the nodes and the backward
+    got = torch.autograd.grad(L1 + L2, (b1, p1, b2, p2))
+    for (wa, wp), ga, gp in ((want[0], got[0], got[1]), (want[1], got[2], got[3])):
+        assert torch.equal(ga, wa) and torch.equal(gp, wp)
+    assert not torch.equal(got[0], got[2])
 
 
 def _smooth_direction(grid, rng, modes=3):
