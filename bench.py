@@ -21,6 +21,7 @@ FP64 Kahan-compensated).  Inputs are resident in HBM when timing starts.
                                                       # CFL and integrals: HBM roofline per entry point
     python bench.py --kernels2d [--n 8192]            # 2D entry points and a 2D band
     python bench.py --band [--n 512]                  # narrow-band step: drop-in chain and fused stages
+    python bench.py --precision-study [--n 512]       # the deformation run over t in [0, 3], FP32 vs FP64
 """
 from __future__ import annotations
 
@@ -596,6 +597,9 @@ def main():
     ap.add_argument("--band", action="store_true",
                     help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
                          "instead of the whole-grid step")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="integrate the deformation workload over t in [0, 3] in FP64 and FP32 (fused stages) and "
+                         "report the FP32 field's L-inf / L1 and volume error against FP64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device copy of 1 GiB before timing (the known-size kernel for HBM counters)")
     a = ap.parse_args()
@@ -637,6 +641,16 @@ def main():
                           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows,
                           "accuracy_fp32_vs_fp64": acc}))
         return
+    if a.precision_study:
+        from lsmlib_amd.solver import precision_study
+        res = precision_study(a.n, 3.0, log=lambda m: print("[bench] precision study " + m, file=sys.stderr,
+                                                            flush=True))
+        print(json.dumps({"metric": "FP32 vs FP64 over t in [0,3] @%d^3" % a.n, "n_gpus": 1, "data": "synthetic",
+                          "config": {"workload": "deformation advection of a sphere r=0.15 at (0.35,0.35,0.35), "
+                                                 "HJ-WENO5 + TVD-RK3 at CFL 0.5 (dt of the unscaled field, last "
+                                                 "step to t = 3), reinitialization every 20 steps, fused stages"},
+                          "result": res}))
+        return
     print("[bench] building the %d^3 deformation workload (FP32 fused)" % a.n, file=sys.stderr, flush=True)
     run32 = DeformationRun(a.n, "f32")
     _stream_ordered(run32.solver.lib)
@@ -656,8 +670,10 @@ def main():
            "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "deformation advection of a sphere r=0.15 at (0.35,0.35,0.35), %d^3 on [0,1]^3, "
-                                  "HJ-WENO5 + TVD-RK3 at CFL 0.5, reinitialization (10 WENO5-RK3 pseudo-steps of "
-                                  "0.5dx) every 20 steps, signed linear extrapolation ghosts" % a.n,
+                                  "HJ-WENO5 + TVD-RK3 at CFL 0.5 (dt fixed at the CFL step of the unscaled "
+                                  "analytic field, evaluated once on the host: no per-step CFL reduction), "
+                                  "reinitialization (10 WENO5-RK3 pseudo-steps of 0.5dx) every 20 steps, signed "
+                                  "linear extrapolation ghosts" % a.n,
                       "grid": a.n, "full_grid_updates_timed": updates, "parallelism": "single GPU"},
            "roofline": roof}
 

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 
@@ -377,8 +378,9 @@ class DeformationRun:
         e1.record()
         self.events.append((kind, e0, e1))
 
-    def advect_step(self):
-        s, phi, (p1, p2), dt, t = self.solver, self.phi, self.work, self.dt, self.t
+    def advect_step(self, dt=None):
+        s, phi, (p1, p2), t = self.solver, self.phi, self.work, self.t
+        dt = self.dt if dt is None else dt
         cs = [deformation_time_factor(t), deformation_time_factor(t + dt), deformation_time_factor(t + 0.5 * dt)]
         if self.chain is None:
             self._stage("advect1", s.stage_deform, p1, phi, None, 1, dt, cs[0])
@@ -416,14 +418,72 @@ class DeformationRun:
             else:
                 self.chain.rk3(self.phi, self.work, self.chain.rhs_reinit, self.dt_r)
 
-    def step(self):
-        """One advection step, plus the reinitialization block when due.
-        Returns the number of full-grid WENO5-RK3 updates performed."""
-        self.advect_step()
+    def step(self, dt=None):
+        """One advection step (of dt, default the CFL step), plus the
+        reinitialization block when due.  Returns the number of full-grid
+        WENO5-RK3 updates performed."""
+        self.advect_step(dt)
         if self.reinit_every and self.nsteps % self.reinit_every == 0:
             self.reinitialize()
             return 1 + self.reinit_steps
         return 1
+
+    def run_to(self, t_end=3.0):
+        """Steps of the CFL step until t_end, the last one shortened to land on
+        t_end (the field's period: the sphere is back at its start).  Returns
+        the number of advection steps."""
+        n = 0
+        while self.t < t_end * (1.0 - 1e-14):
+            self.step(min(self.dt, t_end - self.t))
+            n += 1
+        return n
+
+
+def precision_study(n=512, t_end=3.0, log=None):
+    """The deformation workload integrated over t in [0, t_end] in FP64 and in
+    FP32 through the fused stages, from the same initial field.  Reports the
+    FP32 field's L-inf / L1 difference from the FP64 one (all interior cells,
+    and cells with |phi_64| < 6 dx), the relative difference of the smoothed
+    volume of {phi < 0} (FP64 compensated sum, eps = 1.5 dx), and each
+    precision's error against the initial sphere, which the time-reversed field
+    brings back at t = 3."""
+    import torch
+    out = {"grid": n, "t_end": t_end}
+    finals = {}
+    for prec in ("f64", "f32"):
+        run = DeformationRun(n, prec)
+        t0 = time.perf_counter()
+        steps = run.run_to(t_end)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        run.solver.lib.check()
+        eps = 1.5 * run.grid.dx[0]
+        vol = run.solver.volume_phi_lt0(run.phi, eps)
+        phi = run.grid.interior(run.solver.host(run.phi)).copy()
+        finals[prec] = (phi, vol)
+        out[prec] = {"advection_steps": steps, "seconds": round(el, 2), "volume": vol}
+        if log:
+            log("%s: %d advection steps to t = %g in %.1f s, volume %.15g" % (prec, steps, t_end, el, vol))
+        if prec == "f64":
+            init = run.grid.interior(phi_sphere(run.grid, (0.35, 0.35, 0.35), 0.15))
+            ini = run.solver.field(phi_sphere(run.grid, (0.35, 0.35, 0.35), 0.15))
+            out["volume_initial"] = run.solver.volume_phi_lt0(ini, eps)
+            del ini
+        del run
+        torch.cuda.empty_cache()
+    (p64, v64), (p32, v32) = finals["f64"], finals["f32"]
+    h = 1.0 / n
+    band = np.abs(p64) < 6 * h
+    d = np.abs(p32 - p64)
+    out["fp32_vs_fp64"] = {"linf_phi": float(d.max()), "l1_phi": float(d.mean()),
+                           "linf_phi_band6dx": float(d[band].max()), "l1_phi_band6dx": float(d[band].mean()),
+                           "volume_rel_err": abs(v32 - v64) / abs(v64)}
+    vi = out["volume_initial"]
+    for prec, (p, v) in finals.items():
+        e = np.abs(p - init)
+        out[prec].update({"linf_vs_initial_band6dx": float(e[band].max()), "volume_rel_err_vs_initial":
+                          abs(v - vi) / abs(vi)})
+    return out
 
 
 # ---------------------------------------------------------------------------

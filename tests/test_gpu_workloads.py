@@ -201,3 +201,31 @@ def test_c3_band_full_is_bitwise_the_reference():
     assert bands == g["band_sha256"]
     assert r.dts[0] == float.fromhex(g["dt_first"]) and r.dts[-1] == float.fromhex(g["dt_last"])
     assert _digest(phi) == g["sha256_phi_f64"]
+
+
+def test_c4_precision_study_to_t3_at_128():
+    """The deformation workload integrated over the whole
+    t in [0, 3] at 128^3 -- about 1900 advection steps with the
+    reinitialization every 20 -- in FP64 and FP32 through the fused stages.
+    FP64 fused is bitwise the drop-in chain (tests/test_gpu_fused.py), which
+    is bitwise the oracle.  FP32 against FP64 at t = 3: relative volume of
+    {phi < 0} (FP64 compensated sum) within 1e-5, L-inf within 6 dx of the
+    interface below 1e-4 and L1 below 1e-5; the time-reversed field brings
+    the sphere back: both precisions end within 2 dx of it near the interface
+    with their volumes within 5 % of its volume (sanity bounds; the figures are
+    printed)."""
+    from lsmlib_amd.solver import precision_study
+    r = precision_study(128, 3.0, log=lambda m: print(m, flush=True))
+    c = r["fp32_vs_fp64"]
+    print("C4 128^3 t=3: FP32 vs FP64 Linf %.3e (band %.3e), L1 %.3e (band %.3e), volume rel %.3e; "
+          "vs initial sphere: f64 band Linf %.3e vol %.3e, f32 band Linf %.3e vol %.3e" % (
+              c["linf_phi"], c["linf_phi_band6dx"], c["l1_phi"], c["l1_phi_band6dx"], c["volume_rel_err"],
+              r["f64"]["linf_vs_initial_band6dx"], r["f64"]["volume_rel_err_vs_initial"],
+              r["f32"]["linf_vs_initial_band6dx"], r["f32"]["volume_rel_err_vs_initial"]), flush=True)
+    assert r["f64"]["advection_steps"] == r["f32"]["advection_steps"] > 1000
+    assert c["volume_rel_err"] <= 1e-5
+    assert c["linf_phi_band6dx"] < 1e-4
+    assert c["l1_phi"] < 1e-5
+    for prec in ("f64", "f32"):
+        assert r[prec]["linf_vs_initial_band6dx"] < 2.0 / 128
+        assert r[prec]["volume_rel_err_vs_initial"] < 0.05
