@@ -114,6 +114,15 @@ _RAW = {
     "lsm3dhjweno5local_": "R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:phi B:phi R:D1 B:D1 S:dx S:dy S:dz "
                           "IA:ix IA:iy IA:iz I:nlo0 I:nhi0 I:nlo1 I:nhi1 I:nlo2 I:nhi2 I:nlo3 I:nhi3 M:nb B:nb "
                           "U8:mark_fb U8:mark_D1",
+    "lsm3dcomputegaussiancurvatureorder2local_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g S:dx S:dy "
+                                                 "S:dz IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputemeancurvatureorder4local_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g S:dx S:dy "
+                                             "S:dz IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputegaussiancurvatureorder4local_": "R:kappa B:kappa R:phi B:phi R:px R:py R:pz R:gmag B:g S:dx S:dy "
+                                                 "S:dz IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
+    "lsm3dcomputeorthogonalizationeqnrhslocal_": "R:rhs B:rhs R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm R:psi "
+                                                 "B:psi R:qxp R:qyp R:qzp B:qp R:qxm R:qym R:qzm B:qm S:dx S:dy S:dz "
+                                                 "IA:ix IA:iy IA:iz I:nlo I:nhi M:nb B:nb U8:mark_fb",
     "lsm3dcentralgradorder2local_": "R:px R:py R:pz B:g R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi "
                                     "M:nb B:nb U8:mark_fb",
     "lsm3dcentralgradorder4local_": "R:px R:py R:pz B:g R:phi B:phi S:dx S:dy S:dz IA:ix IA:iy IA:iz I:nlo I:nhi "

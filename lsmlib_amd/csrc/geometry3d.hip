@@ -247,13 +247,7 @@ __device__ __forceinline__ void march(int n, F&& step) {
   }
 }
 
-// Gaussian-curvature numerator, lsm_curvature3d.f.in:230-237 (and :490-497
-// with the x-z mixed derivative taken from phi_z along x).
-__device__ __forceinline__ real_t gauss_num(real_t x, real_t y, real_t z, real_t xx, real_t yy, real_t zz, real_t xy,
-                                            real_t xz, real_t yz) {
-  return x * x * (yy * zz - yz * yz) + y * y * (xx * zz - xz * xz) + z * z * (xx * yy - xy * xy) +
-         real_t(2) * (x * y * (xz * yz - xy * zz) + y * z * (xy * xz - yz * xx) + x * z * (xy * yz - xz * yy));
-}
+using stencil::gauss_num;
 
 struct Factors2 {
   real_t xx, yy, zz, xy, xz, yz;

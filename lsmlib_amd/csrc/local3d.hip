@@ -206,6 +206,94 @@ struct MeanCurvOp {
   }
 };
 
+// Gaussian curvature (second order) on a band, lsm_curvature3d_local.f.in:151-280:
+// the compact second differences of the mean curvature, the Gaussian numerator
+// over |grad phi|^4, 0 where that is below zero_tol.
+struct GaussCurv2Op {
+  CV<real_t> kap;
+  CV<const real_t> F, px, py, pz, gm;
+  real_t fxx, fyy, fzz, fxy, fxz, fyz;
+  __device__ bool operator()(int i, int j, int k) const {
+    if (!kap.has(i, j, k) || !px.has(i, j, k) || !F.star(i, j, k, 1)) return false;
+    const real_t c = F(i, j, k);
+    const real_t xx = (F(i + 1, j, k) - real_t(2) * c + F(i - 1, j, k)) * fxx;
+    const real_t yy = (F(i, j + 1, k) - real_t(2) * c + F(i, j - 1, k)) * fyy;
+    const real_t zz = (F(i, j, k + 1) - real_t(2) * c + F(i, j, k - 1)) * fzz;
+    const real_t xy = (F(i + 1, j + 1, k) - F(i + 1, j - 1, k) - F(i - 1, j + 1, k) + F(i - 1, j - 1, k)) * fxy;
+    const real_t xz = (F(i + 1, j, k + 1) - F(i + 1, j, k - 1) - F(i - 1, j, k + 1) + F(i - 1, j, k - 1)) * fxz;
+    const real_t yz = (F(i, j + 1, k + 1) - F(i, j + 1, k - 1) - F(i, j - 1, k + 1) + F(i, j - 1, k - 1)) * fyz;
+    const real_t g = gm(i, j, k), den = g * g * g * g;
+    kap(i, j, k) = den < kZeroTol ? real_t(0)
+                                  : stencil::gauss_num(px(i, j, k), py(i, j, k), pz(i, j, k), xx, yy, zz, xy, xz, yz) /
+                                        den;
+    return true;
+  }
+};
+
+// The fourth-order curvature of lsm_curvature3d_local.f.in:285-426 and :432-573:
+// fourth-order second differences of phi, mixed derivatives as fourth-order
+// differences of the given first derivatives (phi_xy = d/dy phi_x, phi_yz =
+// d/dz phi_y, phi_zx = d/dx phi_z), the Gaussian numerator over |grad phi|^4.
+// Both routines of the reference compute this same expression (the "mean"
+// one included), and so do both entry points here.
+struct Curv4Op {
+  CV<real_t> kap;
+  CV<const real_t> F, px, py, pz, gm;
+  real_t f2x, f2y, f2z, f1x, f1y, f1z;
+  __device__ bool operator()(int i, int j, int k) const {
+    if (!kap.has(i, j, k) || !F.star(i, j, k, 2) || !px.star(i, j, k, 2)) return false;
+    const real_t c = F(i, j, k);
+    const real_t xx =
+        (-F(i + 2, j, k) + real_t(16) * F(i + 1, j, k) - real_t(30) * c - F(i - 2, j, k) + real_t(16) * F(i - 1, j, k)) *
+        f2x;
+    const real_t yy =
+        (-F(i, j + 2, k) + real_t(16) * F(i, j + 1, k) - real_t(30) * c - F(i, j - 2, k) + real_t(16) * F(i, j - 1, k)) *
+        f2y;
+    const real_t zz =
+        (-F(i, j, k + 2) + real_t(16) * F(i, j, k + 1) - real_t(30) * c - F(i, j, k - 2) + real_t(16) * F(i, j, k - 1)) *
+        f2z;
+    const real_t xy =
+        (-px(i, j + 2, k) + real_t(8) * px(i, j + 1, k) + px(i, j - 2, k) - real_t(8) * px(i, j - 1, k)) * f1y;
+    const real_t yz =
+        (-py(i, j, k + 2) + real_t(8) * py(i, j, k + 1) + py(i, j, k - 2) - real_t(8) * py(i, j, k - 1)) * f1z;
+    const real_t zx =
+        (-pz(i + 2, j, k) + real_t(8) * pz(i + 1, j, k) + pz(i - 2, j, k) - real_t(8) * pz(i - 1, j, k)) * f1x;
+    const real_t g = gm(i, j, k), den = g * g * g * g;
+    kap(i, j, k) = den < kZeroTol ? real_t(0)
+                                  : stencil::gauss_num(px(i, j, k), py(i, j, k), pz(i, j, k), xx, yy, zz, xy, zx, yz) /
+                                        den;
+    return true;
+  }
+};
+
+// Orthogonalization RHS on a band, lsm_reinitialization3d_local.f.in:326-522
+// (the whole-grid formula of lsm_reinitialization3d.f.in:296-476 at each point).
+struct OrthoOp {
+  CV<real_t> rhs;
+  CV<const real_t> xp, yp, zp, xm, ym, zm, psi, qxp, qyp, qzp, qxm, qym, qzm;
+  real_t dx_sq;
+  __device__ bool operator()(int i, int j, int k) const {
+    if (!rhs.has(i, j, k) || !xp.has(i, j, k) || !xm.has(i, j, k) || !psi.has(i, j, k) || !qxp.has(i, j, k) ||
+        !qxm.has(i, j, k))
+      return false;
+    const real_t g1 = real_t(0.5) * (qxp(i, j, k) + qxm(i, j, k));
+    const real_t g2 = real_t(0.5) * (qyp(i, j, k) + qym(i, j, k));
+    const real_t g3 = real_t(0.5) * (qzp(i, j, k) + qzm(i, j, k));
+    const real_t ng = sqrt(g1 * g1 + g2 * g2 + g3 * g3);
+    const real_t s = psi(i, j, k);
+    real_t out = real_t(0);
+    if (fabs(s) > kZeroTol && ng > kZeroTol) {
+      const real_t sgn = s / sqrt(s * s + dx_sq * (ng * ng));
+      const real_t a1 = sgn * g1, a2 = sgn * g2, a3 = sgn * g3, z = real_t(0);
+      out = real_t(-1) / ng *
+            (maxr(a1, z) * xm(i, j, k) + minr(a1, z) * xp(i, j, k) + maxr(a2, z) * ym(i, j, k) +
+             minr(a2, z) * yp(i, j, k) + maxr(a3, z) * zm(i, j, k) + minr(a3, z) * zp(i, j, k));
+    }
+    rhs(i, j, k) = out;
+    return true;
+  }
+};
+
 struct Central2Op {
   Axis3 out;
   CV<const real_t> F;
@@ -595,6 +683,72 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2local_(real_t* kappa, LSMHIP_B
                 MeanCurvOp{cv(kappa, LSMHIP_BOX(kg)), cv(phi, LSMHIP_BOX(pg)), cv(px, gg), cv(py, gg), cv(pz, gg),
                            cv(gmag, gg), real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
                            real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz});
+  finish_local(who);
+}
+
+LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder2local_(real_t* kappa, LSMHIP_BOXP(kg), const real_t* phi,
+                                                             LSMHIP_BOXP(pg), const real_t* px, const real_t* py,
+                                                             const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
+                                                             const real_t* dx, const real_t* dy, const real_t* dz,
+                                                             LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag, ix, iy, iz, nb);
+  const char* who = "lsm3dcomputegaussiancurvatureorder2local_";
+  const Box gg = LSMHIP_BOX(gg);
+  launch_points(who, LIST, FBMASK,
+                GaussCurv2Op{cv(kappa, LSMHIP_BOX(kg)), cv(phi, LSMHIP_BOX(pg)), cv(px, gg), cv(py, gg), cv(pz, gg),
+                             cv(gmag, gg), real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
+                             real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz});
+  finish_local(who);
+}
+
+namespace {
+void curv4_local(const char* who, real_t* kappa, const Box& kg, const real_t* phi, const Box& pg, const real_t* px,
+                 const real_t* py, const real_t* pz, const real_t* gmag, const Box& gg, real_t dx, real_t dy,
+                 real_t dz, const IndexList& L, const Mask& M) {
+  const real_t c12 = real_t(0.0833333333333333333333);
+  launch_points(who, L, M,
+                Curv4Op{cv(kappa, kg), cv(phi, pg), cv(px, gg), cv(py, gg), cv(pz, gg), cv(gmag, gg), c12 / dx / dx,
+                        c12 / dy / dy, c12 / dz / dz, c12 / dx, c12 / dy, c12 / dz});
+  finish_local(who);
+}
+}  // namespace
+
+// The reference's lsm3dComputeMeanCurvatureOrder4Local computes the Gaussian
+// curvature of lsm3dComputeGaussianCurvatureOrder4Local (identical bodies,
+// lsm_curvature3d_local.f.in:352-418 and :499-565); kept as it is.
+LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder4local_(real_t* kappa, LSMHIP_BOXP(kg), const real_t* phi,
+                                                         LSMHIP_BOXP(pg), const real_t* px, const real_t* py,
+                                                         const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
+                                                         const real_t* dx, const real_t* dy, const real_t* dz,
+                                                         LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag, ix, iy, iz, nb);
+  curv4_local("lsm3dcomputemeancurvatureorder4local_", kappa, LSMHIP_BOX(kg), phi, LSMHIP_BOX(pg), px, py, pz, gmag,
+              LSMHIP_BOX(gg), *dx, *dy, *dz, LIST, FBMASK);
+}
+LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder4local_(real_t* kappa, LSMHIP_BOXP(kg), const real_t* phi,
+                                                             LSMHIP_BOXP(pg), const real_t* px, const real_t* py,
+                                                             const real_t* pz, const real_t* gmag, LSMHIP_BOXP(gg),
+                                                             const real_t* dx, const real_t* dy, const real_t* dz,
+                                                             LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(kappa, phi, px, py, pz, gmag, ix, iy, iz, nb);
+  curv4_local("lsm3dcomputegaussiancurvatureorder4local_", kappa, LSMHIP_BOX(kg), phi, LSMHIP_BOX(pg), px, py, pz,
+              gmag, LSMHIP_BOX(gg), *dx, *dy, *dz, LIST, FBMASK);
+}
+
+LSMHIP_EXPORT void lsm3dcomputeorthogonalizationeqnrhslocal_(
+    real_t* rhs, LSMHIP_BOXP(rg), const real_t* pxp, const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
+    const real_t* pxm, const real_t* pym, const real_t* pzm, LSMHIP_BOXP(gm), const real_t* psi, LSMHIP_BOXP(sg),
+    const real_t* qxp, const real_t* qyp, const real_t* qzp, LSMHIP_BOXP(qp), const real_t* qxm, const real_t* qym,
+    const real_t* qzm, LSMHIP_BOXP(qm), const real_t* dx, const real_t* dy, const real_t* dz, LOCAL_TAIL) {
+  LSMHIP_REQUIRE_ARRAYS(rhs, pxp, pyp, pzp, pxm, pym, pzm, psi, qxp, qyp, qzp, qxm, qym, qzm, ix, iy, iz, nb);
+  const char* who = "lsm3dcomputeorthogonalizationeqnrhslocal_";
+  const Box gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm), qp = LSMHIP_BOX(qp), qm = LSMHIP_BOX(qm);
+  real_t h = *dx > *dy ? *dx : *dy;
+  h = h > *dz ? h : *dz;
+  launch_points(who, LIST, FBMASK,
+                OrthoOp{cv(rhs, LSMHIP_BOX(rg)), cv(pxp, gp), cv(pyp, gp), cv(pzp, gp), cv(pxm, gm), cv(pym, gm),
+                        cv(pzm, gm), cv(psi, LSMHIP_BOX(sg)), cv(qxp, qp), cv(qyp, qp), cv(qzp, qp), cv(qxm, qm),
+                        cv(qym, qm), cv(qzm, qm), h * h});
   finish_local(who);
 }
 

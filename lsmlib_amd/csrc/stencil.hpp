@@ -212,6 +212,16 @@ __device__ __forceinline__ T curv_num(T xx, T xy, T xz, T yy, T yz, T zz, T x, T
          zz * y * y - T(2) * yz * y * z;
 }
 
+// Gaussian-curvature numerator, lsm_curvature3d.f.in:230-237 (and :490-497
+// with the x-z mixed derivative taken from phi_z along x); the narrow-band
+// routines of lsm_curvature3d_local.f.in:258-264, 408-414, 555-561 use the same
+// expression.
+template <class T>
+__device__ __forceinline__ T gauss_num(T x, T y, T z, T xx, T yy, T zz, T xy, T xz, T yz) {
+  return x * x * (yy * zz - yz * yz) + y * y * (xx * zz - xz * xz) + z * z * (xx * yy - xy * xy) +
+         T(2) * (x * y * (xz * yz - xy * zz) + y * z * (xy * xz - yz * xx) + x * z * (xy * yz - xz * yy));
+}
+
 // x^1.5 for x >= 1e-11.  FP64: the exact product x * sqrt(x) carried as a
 // double-double (sqrt's remainder x - s^2 and the product's rounding error are
 // both exact with FMA) and rounded once, i.e. correctly rounded except within

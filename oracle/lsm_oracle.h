@@ -86,6 +86,18 @@ void orc_bc_face(orc_arr P, obox gb, obox fb, int idx, int kind);
 /* Fortran integer*1 marks are compared as signed bytes. */
 #define MARK(a, i, j, k) ((signed char)AT(a, i, j, k))
 
+/* Index lists are addressed as the reference's Fortran declarations address
+ * them: index_x(nlo:nhi) is passed as a pointer to its element nlo. */
+#define LIDX(p, lo, l) ((p)[(l) - (lo)])
+
+/* The band loop of every 3D narrow-band routine: points nlo..nhi of the index
+ * lists ix, iy, iz (in scope), those whose mark is <= mark_fb. */
+#define FOR_BAND(lo, hi, nbv, mark)                                             \
+  for (int l = (lo); l <= (hi); l++)                                            \
+    for (int i = LIDX(ix, (lo), l), j = LIDX(iy, (lo), l), k = LIDX(iz, (lo), l), \
+             _once = 1;                                                          \
+         _once && MARK(nbv, i, j, k) <= (signed char)(mark); _once = 0)
+
 static inline double orc_sq(double x) { return x * x; }
 static inline double orc_max(double a, double b) { return a > b ? a : b; }
 static inline double orc_min(double a, double b) { return a < b ? a : b; }

@@ -275,6 +275,115 @@ void oracle_lsm3dcomputeorthogonalizationeqnrhs(double *rhs, BOXP(rg), const dou
 }
 
 /* ---------------------------------------------------------------------------
+ * Narrow-band twins, src/geometry/lsm_curvature3d_local.f.in and
+ * src/reinitialization/lsm_reinitialization3d_local.f.in: the whole-grid
+ * formulas above at the band points nlo..nhi whose mark is <= mark_fb.
+ * ------------------------------------------------------------------------- */
+#define BAND_TAIL                                                                                   \
+  const int *ix, const int *iy, const int *iz, const int *nlo, const int *nhi, const unsigned char *nb, \
+      BOXP(nbg), const unsigned char *mark_fb
+
+/* lsm3dComputeGaussianCurvatureOrder2Local, lsm_curvature3d_local.f.in:151-280 */
+void oracle_lsm3dcomputegaussiancurvatureorder2local(double *kappa, BOXP(kg), const double *phi, BOXP(pg),
+                                                     const double *px, const double *py, const double *pz,
+                                                     const double *gmag, BOXP(gg), const double *dx,
+                                                     const double *dy, const double *dz, BAND_TAIL) {
+  orc_arr K = VIEW(kappa, kg), F = VIEW(phi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg),
+          G = VIEW(gmag, gg);
+  orc_mask N = MVIEW(nb, nbg);
+  double fxx = 1.0 / *dx / *dx, fyy = 1.0 / *dy / *dy, fzz = 1.0 / *dz / *dz;
+  double fxy = 0.25 / *dx / *dy, fxz = 0.25 / *dx / *dz, fyz = 0.25 / *dy / *dz;
+  FOR_BAND(*nlo, *nhi, N, *mark_fb) {
+    double c = AT(F, i, j, k);
+    double xx = (AT(F, i + 1, j, k) - 2.0 * c + AT(F, i - 1, j, k)) * fxx;
+    double yy = (AT(F, i, j + 1, k) - 2.0 * c + AT(F, i, j - 1, k)) * fyy;
+    double zz = (AT(F, i, j, k + 1) - 2.0 * c + AT(F, i, j, k - 1)) * fzz;
+    double xy = (AT(F, i + 1, j + 1, k) - AT(F, i + 1, j - 1, k) - AT(F, i - 1, j + 1, k) +
+                 AT(F, i - 1, j - 1, k)) * fxy;
+    double xz = (AT(F, i + 1, j, k + 1) - AT(F, i + 1, j, k - 1) - AT(F, i - 1, j, k + 1) +
+                 AT(F, i - 1, j, k - 1)) * fxz;
+    double yz = (AT(F, i, j + 1, k + 1) - AT(F, i, j + 1, k - 1) - AT(F, i, j - 1, k + 1) +
+                 AT(F, i, j - 1, k - 1)) * fyz;
+    double g = AT(G, i, j, k), den = g * g * g * g;
+    if (den < ORC_ZERO_TOL)
+      AT(K, i, j, k) = 0.0;
+    else
+      AT(K, i, j, k) = gauss_num(AT(X, i, j, k), AT(Y, i, j, k), AT(Z, i, j, k), xx, yy, zz, xy, xz, yz) / den;
+  }
+}
+
+/* lsm3dComputeMeanCurvatureOrder4Local (:285-426) and
+ * lsm3dComputeGaussianCurvatureOrder4Local (:432-573) evaluate the same
+ * expression, the Gaussian numerator over |grad(phi)|^4 with the fourth-order
+ * Hessian, so both symbols below run this one loop. */
+static void curv4_band(double *kappa, BOXP(kg), const double *phi, BOXP(pg), const double *px, const double *py,
+                       const double *pz, const double *gmag, BOXP(gg), const double *dx, const double *dy,
+                       const double *dz, BAND_TAIL) {
+  orc_arr K = VIEW(kappa, kg), F = VIEW(phi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg),
+          G = VIEW(gmag, gg);
+  orc_mask N = MVIEW(nb, nbg);
+  ORDER4_FACTORS;
+  FOR_BAND(*nlo, *nhi, N, *mark_fb) {
+    hess4 h = hessian_order4(F, X, Y, Z, i, j, k, f2, f1);
+    double g = AT(G, i, j, k), den = g * g * g * g;
+    if (den < ORC_ZERO_TOL)
+      AT(K, i, j, k) = 0.0;
+    else
+      AT(K, i, j, k) =
+          gauss_num(AT(X, i, j, k), AT(Y, i, j, k), AT(Z, i, j, k), h.xx, h.yy, h.zz, h.xy, h.zx, h.yz) / den;
+  }
+}
+
+#define CURV4_FORWARD                                                                                     \
+  curv4_band(kappa, ilo_kg, ihi_kg, jlo_kg, jhi_kg, klo_kg, khi_kg, phi, ilo_pg, ihi_pg, jlo_pg, jhi_pg, klo_pg, \
+             khi_pg, px, py, pz, gmag, ilo_gg, ihi_gg, jlo_gg, jhi_gg, klo_gg, khi_gg, dx, dy, dz, ix, iy, iz,   \
+             nlo, nhi, nb, ilo_nbg, ihi_nbg, jlo_nbg, jhi_nbg, klo_nbg, khi_nbg, mark_fb)
+
+void oracle_lsm3dcomputemeancurvatureorder4local(double *kappa, BOXP(kg), const double *phi, BOXP(pg),
+                                                 const double *px, const double *py, const double *pz,
+                                                 const double *gmag, BOXP(gg), const double *dx, const double *dy,
+                                                 const double *dz, BAND_TAIL) {
+  CURV4_FORWARD;
+}
+
+void oracle_lsm3dcomputegaussiancurvatureorder4local(double *kappa, BOXP(kg), const double *phi, BOXP(pg),
+                                                     const double *px, const double *py, const double *pz,
+                                                     const double *gmag, BOXP(gg), const double *dx,
+                                                     const double *dy, const double *dz, BAND_TAIL) {
+  CURV4_FORWARD;
+}
+
+/* lsm3dComputeOrthogonalizationEqnRHSLocal, lsm_reinitialization3d_local.f.in:326-522 */
+void oracle_lsm3dcomputeorthogonalizationeqnrhslocal(
+    double *rhs, BOXP(rg), const double *pxp, const double *pyp, const double *pzp, BOXP(gp), const double *pxm,
+    const double *pym, const double *pzm, BOXP(gm), const double *psi, BOXP(sg), const double *qxp,
+    const double *qyp, const double *qzp, BOXP(qp), const double *qxm, const double *qym, const double *qzm,
+    BOXP(qm), const double *dx, const double *dy, const double *dz, BAND_TAIL) {
+  orc_arr R = VIEW(rhs, rg), XP = VIEW(pxp, gp), YP = VIEW(pyp, gp), ZP = VIEW(pzp, gp), XM = VIEW(pxm, gm),
+          YM = VIEW(pym, gm), ZM = VIEW(pzm, gm), S = VIEW(psi, sg), QXP = VIEW(qxp, qp), QYP = VIEW(qyp, qp),
+          QZP = VIEW(qzp, qp), QXM = VIEW(qxm, qm), QYM = VIEW(qym, qm), QZM = VIEW(qzm, qm);
+  orc_mask N = MVIEW(nb, nbg);
+  double h = max3(*dx, *dy, *dz), dx_sq = h * h;
+  FOR_BAND(*nlo, *nhi, N, *mark_fb) {
+    double g1 = 0.5 * (AT(QXP, i, j, k) + AT(QXM, i, j, k));
+    double g2 = 0.5 * (AT(QYP, i, j, k) + AT(QYM, i, j, k));
+    double g3 = 0.5 * (AT(QZP, i, j, k) + AT(QZM, i, j, k));
+    double ng = sqrt(g1 * g1 + g2 * g2 + g3 * g3);
+    double s = AT(S, i, j, k);
+    if (fabs(s) > ORC_ZERO_TOL && ng > ORC_ZERO_TOL) {
+      double sgn = s / sqrt(s * s + dx_sq * (ng * ng));
+      double a1 = sgn * g1, a2 = sgn * g2, a3 = sgn * g3;
+      AT(R, i, j, k) = -1.0 / ng *
+                       (orc_max(a1, 0.0) * AT(XM, i, j, k) + orc_min(a1, 0.0) * AT(XP, i, j, k) +
+                        orc_max(a2, 0.0) * AT(YM, i, j, k) + orc_min(a2, 0.0) * AT(YP, i, j, k) +
+                        orc_max(a3, 0.0) * AT(ZM, i, j, k) + orc_min(a3, 0.0) * AT(ZP, i, j, k));
+    } else {
+      AT(R, i, j, k) = 0.0;
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------------
  * Field extension RHS, src/field_extension/lsm_field_extension3d.f:31-156
  * ------------------------------------------------------------------------- */
 

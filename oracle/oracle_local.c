@@ -8,8 +8,6 @@
  */
 #include "lsm_oracle.h"
 
-#define LIDX(p, lo, l) ((p)[(l) - (lo)])
-
 /* lsm3dComputeDnLOCAL, src/toolbox/lsm_spatial_derivatives3d_local.f.in:43-138 */
 static void computeDnLocal(orc_arr Dn, orc_arr Dm1, int n, int dir, const int *ix, const int *iy,
                            const int *iz, int lo, int hi, int base, orc_mask nb, signed char mark) {
@@ -158,12 +156,6 @@ void oracle_lsm3dhjweno5local(double *pxp, double *pyp, double *pzp, BOXP(gp), d
     }
   }
 }
-
-#define FOR_BAND(lo, hi, nbv, mark)                                             \
-  for (int l = (lo); l <= (hi); l++)                                            \
-    for (int i = LIDX(ix, (lo), l), j = LIDX(iy, (lo), l), k = LIDX(iz, (lo), l), \
-             _once = 1;                                                          \
-         _once && MARK(nbv, i, j, k) <= (signed char)(mark); _once = 0)
 
 /* lsm3dCentralGradOrder2LOCAL, :763-846 */
 void oracle_lsm3dcentralgradorder2local(double *px, double *py, double *pz, BOXP(g), const double *phi,

@@ -10,7 +10,6 @@
 #include "lsm_oracle.h"
 
 #define BOXP2(n) const int *ilo_##n, const int *ihi_##n, const int *jlo_##n, const int *jhi_##n
-#define LIDX(p, lo, l) ((p)[(l) - (lo)])
 
 static orc_arr v2(const double *p, const int *ilo, const int *ihi, const int *jlo, const int *jhi) {
   orc_arr a;
