@@ -89,6 +89,12 @@ int lsmhip_deform_velocity_fill(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_R
                                 LSMHIP_BOX_ARGS(b), const LSMLIB_REAL *const *tabs, LSMLIB_REAL c);
 /* k planes a fused-stage block marches through (default 64). */
 void lsmhip_set_fused_kchunk(int planes);
+/* FP32 library: 2 (default) selects the stage kernels that form each
+ * position's WENO5 terms once and share them between neighbouring cells, 1 the
+ * kernels that form a window per cell (the same scheme with different
+ * rounding, kept for comparison).  Returns the previous choice; ignored by the
+ * FP64 library. */
+int lsmhip_set_fused_f32_kernels(int which);
 
 /* ---- fused narrow-band stages: one TVD-RK stage of normal motion on a band ----
  * The band as the reference's local driver builds it (LSM3D_DETERMINE_NARROW_BAND

@@ -367,11 +367,450 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 advection stages with the WENO5 inputs shared between neighbouring
+// cells.
+//
+// Along an axis, write D(m) = u(m) - u(m-1) and, for the three consecutive
+// differences around position m, A(m) = D(m-1) - 2 D(m) + D(m+1),
+// C(m) = D(m-1) - D(m+1) and T(m) = 13/3 A(m)^2.  The minus-side
+// reconstruction at m, weno5(D(m-2), .., D(m+2)), needs (4 S_k = 13/3 a_k^2 +
+// b_k^2, the scaling of stencil::fastf::win5)
+//   4 S1 = T(m-1) + (2 A(m-1) - C(m-1))^2,  4 S2 = T(m) + C(m)^2,
+//   4 S3 = T(m+1) + (2 A(m+1) + C(m+1))^2,
+//   a1 - a2 = A(m-1) - A(m),  a2 - a3 = A(m) - A(m+1),
+//   p2 = D(m) + A(m)/12 - C(m)/4,
+// and the plus-side one, weno5(D(m+3), .., D(m-1)), the same terms at
+// m+2, m+1, m with the sign of C reversed.  So each position's D, A, C, T are
+// formed once and read by every cell whose stencil holds them: along z from
+// registers carried from plane to plane, along x and y from LDS planes (x: each
+// wave forms its own row, the five positions past the tile by a few lanes;
+// y: the tile's rows and the rows past it, shared out over the waves).  The
+// upwind side is a branch rather than a per-lane select: waves whose cells all
+// see the same sign along an axis (nearly all of them for a smooth velocity)
+// evaluate one side only.  Same weights and candidates as the other FP32
+// forms (stencil::fastf), different rounding.
+// ---------------------------------------------------------------------------
+constexpr int AXD = TX + 5;  // D along x at i0-2 .. i0+TX+2
+constexpr int AXQ = TX + 3;  // A, C, T along x at i0-1 .. i0+TX+1
+constexpr int AYD = TY + 5;  // D along y at rows j0-2 .. j0+TY+2
+constexpr int AYQ = TY + 3;  // A, C, T along y at rows j0-1 .. j0+TY+1
+
+struct Quad {  // A, C, T of one position
+  float A, C, T;
+};
+__device__ __forceinline__ Quad quad(float dm, float d0, float dp) {
+  const float A = fmaf(-2.f, d0, dm) + dp;
+  return Quad{A, dm - dp, (13.f / 3.f) * A * A};
+}
+
+// weno5 from 4 S_k (unnormalised), the candidate differences d12 = a1 - a2,
+// d23 = a2 - a3, the middle candidate p2 and ma = max |v|: win5 + fwd of
+// stencil::fastf with the terms given.  With Q_k = e_k^2, the weighted sum
+//   p2 + (0.1 Q2Q3 d12/3 + 0.3 Q1Q2 d23/6) / (0.1 Q2Q3 + 0.6 Q1Q3 + 0.3 Q1Q2)
+// is formed as p2 + Q2 (Q3 d12/3 + Q1 d23/2) / (Q2 (Q3 + 3 Q1) + 6 Q1 Q3).
+__device__ __forceinline__ float weno_core(float E1, float E2, float E3, float d12, float d23, float p2, float ma) {
+  using stencil::fastf::rcp;
+  const float im = rcp(fmaf(ma, ma, 1e-30f));
+  const float e1 = fmaf(E1, im, 4e-6f), e2 = fmaf(E2, im, 4e-6f), e3 = fmaf(E3, im, 4e-6f);
+  const float Q1 = e1 * e1, Q2 = e2 * e2, Q3 = e3 * e3;
+  const float num = Q2 * fmaf(Q3 * d12, 1.f / 3.f, 0.5f * (Q1 * d23));
+  const float den = fmaf(6.f * Q1, Q3, Q2 * fmaf(3.f, Q1, Q3));
+  return fmaf(num, rcp(den), p2);
+}
+
+// max |v| over five values as a chain (two v_max3 with absolute-value operands)
+__device__ __forceinline__ float amax5(float a, float b, float c, float d, float e) {
+  return fmaxf(fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(c)), fabsf(d)), fabsf(e));
+}
+
+// The upwinded reconstruction (in undivided units) at position m of one axis
+// from D(m-2..m+3) (d[0..5]) and the quads at m-1..m+2 (q[0..3]): the minus
+// side for vel > 0, the plus side otherwise, 0 when |vel| < tol.  Each side
+// finishes its own reconstruction, so a wave of one sign runs one side only.
+__device__ __forceinline__ float upwind_q(float vel, float tol, const float d[6], const Quad q[4]) {
+  float r = 0.f;
+  if (!(fabsf(vel) < tol)) {
+    if (vel > 0.f) {
+      const float b1 = fmaf(2.f, q[0].A, -q[0].C), b3 = fmaf(2.f, q[2].A, q[2].C);
+      r = weno_core(fmaf(b1, b1, q[0].T), fmaf(q[1].C, q[1].C, q[1].T), fmaf(b3, b3, q[2].T), q[0].A - q[1].A,
+                    q[1].A - q[2].A, fmaf(-0.25f, q[1].C, fmaf(1.f / 12.f, q[1].A, d[2])),
+                    amax5(d[0], d[1], d[2], d[3], d[4]));
+    } else {
+      const float b1 = fmaf(2.f, q[3].A, q[3].C), b3 = fmaf(2.f, q[1].A, -q[1].C);
+      r = weno_core(fmaf(b1, b1, q[3].T), fmaf(q[2].C, q[2].C, q[2].T), fmaf(b3, b3, q[1].T), q[3].A - q[2].A,
+                    q[2].A - q[1].A, fmaf(0.25f, q[2].C, fmaf(1.f / 12.f, q[2].A, d[3])),
+                    amax5(d[1], d[2], d[3], d[4], d[5]));
+    }
+  }
+  return r;
+}
+
+template <int RHS, int STAGE, class F>
+__global__ void __launch_bounds__(TX* TY) fused_advect_f32_kernel(FusedArgs a) {
+  __shared__ F plane[PLANE];       // u at plane k with a 3-cell halo, [py][px]
+  __shared__ F dxs[TY][AXD];       // D along x, index i - i0 + 2
+  __shared__ Quad qxs[TY][AXQ];    // quads along x, index i - i0 + 1
+  __shared__ F dys[AYD][TX];       // D along y, row j - j0 + 2
+  __shared__ Quad qys[AYQ][TX];    // quads along y, row j - j0 + 1
+  static_assert(AYD - TY <= TY && AYQ - TY <= TY && TY * (AXD - TX) <= TX && TY * (AXQ - TX) <= TX && TY >= 7,
+                "one pass of waves covers the rows past the tile; waves 5 and 6 the positions past each row");
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int nt = a.nti * a.ntj * a.ntk;
+  const int b = blockIdx.x;
+  const int per = (nt + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= nt) return;
+  const int ti = t % a.nti, tj = (t / a.nti) % a.ntj, tk = t / (a.nti * a.ntj);
+  const int i0 = a.fb.ilo + ti * TX, j0 = a.fb.jlo + tj * TY;
+  const int kb = a.fb.klo + tk * a.kc;
+  const int ke = min(kb + a.kc - 1, a.fb.khi);
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int i = i0 + lx, j = j0 + ly;
+  const Box gb = a.gb;
+  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
+  const bool own_in = i <= gb.ihi && j <= gb.jhi;
+  const long long own = (long long)(i - gb.ilo) + (long long)a.sy * (j - gb.jlo);
+  long long hoff = -1;
+  int hslot = 0;
+  if (tid < NHALO) {
+    int px, py;
+    halo_pos(tid, px, py);
+    const int gi = i0 - H + px, gj = j0 - H + py;
+    hslot = py * PX + px;
+    if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
+      hoff = (long long)(gi - gb.ilo) + (long long)a.sy * (gj - gb.jlo);
+  }
+  const F* us = a.us;
+  const long long sz = a.sz;
+  // positions past each row along x: wave 5 forms D at dxs index 0, 1 (i0-2,
+  // i0-1) and TX+2 .. TX+4 of every row (lane = 5 row + slot), wave 6 the
+  // quads at qxs index 0, TX+1, TX+2 (lane = 3 row + slot)
+  const int hd_row = lx / (AXD - TX), hd_slot = lx % (AXD - TX);
+  const int hd = hd_slot < 2 ? hd_slot : TX + hd_slot;
+  const int hq_row = lx / (AXQ - TX), hq_slot = lx % (AXQ - TX);
+  const int hq = hq_slot == 0 ? 0 : TX + hq_slot;
+  const bool hd_lane = ly == 5 && lx < TY * (AXD - TX), hq_lane = ly == 6 && lx < TY * (AXQ - TX);
+
+  // z: u at k..k+2, D at k-2..k+3 (dz[s+2] = D(k+s)), quads at k-1..k+2 (qz[s+1])
+  F u0, u1, u2;
+  F dz[6];
+  Quad qz[4];
+  {
+    F w[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) w[s] = own_in ? us[own + sz * (kb - H + s - gb.klo)] : F(0);  // k-3 .. k+2
+#pragma unroll
+    for (int s = 0; s < 5; s++) dz[s] = w[s + 1] - w[s];  // D(k-2 .. k+2)
+#pragma unroll
+    for (int s = 0; s < 3; s++) qz[s] = quad(dz[s], dz[s + 1], dz[s + 2]);  // k-1 .. k+1
+    u0 = w[3];
+    u1 = w[4];
+    u2 = w[5];
+  }
+  F ta0 = 0, tb0 = 0, ta1 = 0, tb1 = 0;
+  if (RHS == kAdvectDeform && active) {
+    ta0 = a.ta[0][i - gb.ilo];
+    tb0 = a.tb[0][i - gb.ilo];
+    ta1 = a.ta[1][j - gb.jlo];
+    tb1 = a.tb[1][j - gb.jlo];
+  }
+  // the deformation velocity divided by the grid spacing: u/dx = X0 B(z), v/dy = X1 B(z), w/dz = X2 A(z)
+  const F X0 = F(2) * a.tfac * a.ix * ta0 * tb1, X1 = -(a.tfac * a.iy) * tb0 * ta1, X2 = -(a.tfac * a.iz) * tb0 * tb1;
+  const F tolx = kZeroTol * a.ix, toly = kZeroTol * a.iy, tolz = kZeroTol * a.iz;
+
+  for (int k = kb; k <= ke; k++) {
+    const long long zoff = sz * (k - gb.klo);
+    const F u3 = own_in ? us[own + zoff + H * sz] : F(0);
+    const F hv = hoff >= 0 ? us[hoff + zoff] : F(0);
+    const long long off = own + zoff;
+    const F ucv = STAGE != 1 && active ? a.uc[off] : F(0);
+    F c0 = F(0), c1 = F(0), c2 = F(0);
+    if (RHS == kAdvectField && active) {
+      c0 = a.vx[off];
+      c1 = a.vy[off];
+      c2 = a.vz[off];
+    }
+    __syncthreads();  // the previous plane's LDS reads are done
+    plane[(ly + H) * PX + lx + H] = u0;
+    if (tid < NHALO) plane[hslot] = hv;
+    // z: D(k+3) and the quad at k+2
+    dz[5] = u3 - u2;
+    qz[3] = quad(dz[3], dz[4], dz[5]);
+    __syncthreads();
+    // D along x (own row) and along y (row ly, and the rows past the tile)
+    dxs[ly][lx + 2] = plane[(ly + H) * PX + lx + H] - plane[(ly + H) * PX + lx + H - 1];
+    if (hd_lane) dxs[hd_row][hd] = plane[(hd_row + H) * PX + hd + 1] - plane[(hd_row + H) * PX + hd];
+    dys[ly][lx] = plane[(ly + 1) * PX + lx + H] - plane[ly * PX + lx + H];
+    if (ly < AYD - TY) dys[ly + TY][lx] = plane[(ly + TY + 1) * PX + lx + H] - plane[(ly + TY) * PX + lx + H];
+    __syncthreads();
+    qxs[ly][lx + 1] = quad(dxs[ly][lx + 1], dxs[ly][lx + 2], dxs[ly][lx + 3]);
+    if (hq_lane) qxs[hq_row][hq] = quad(dxs[hq_row][hq], dxs[hq_row][hq + 1], dxs[hq_row][hq + 2]);
+    qys[ly][lx] = quad(dys[ly][lx], dys[ly + 1][lx], dys[ly + 2][lx]);
+    if (ly < AYQ - TY) qys[ly + TY][lx] = quad(dys[ly + TY][lx], dys[ly + TY + 1][lx], dys[ly + TY + 2][lx]);
+    __syncthreads();
+    if (active) {
+      F ux, vy, wz;
+      if (RHS == kAdvectDeform) {
+        ux = X0 * a.tb[2][k - gb.klo];
+        vy = X1 * a.tb[2][k - gb.klo];
+        wz = X2 * a.ta[2][k - gb.klo];
+      } else {
+        ux = c0 * a.ix;
+        vy = c1 * a.iy;
+        wz = c2 * a.iz;
+      }
+      F dx6[6], dy6[6];
+      Quad qx4[4], qy4[4];
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        dx6[s] = dxs[ly][lx + s];
+        dy6[s] = dys[ly + s][lx];
+      }
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        qx4[s] = qxs[ly][lx + s];
+        qy4[s] = qys[ly + s][lx];
+      }
+      const F px = upwind_q(ux, tolx, dx6, qx4);
+      const F py = upwind_q(vy, toly, dy6, qy4);
+      const F pz = upwind_q(wz, tolz, dz, qz);
+      // r = -(u phi_x + v phi_y + w phi_z), the velocities already divided by the spacing
+      const F r = F(0) - (ux * px + vy * py + wz * pz);
+      const F uc = STAGE == 1 ? u0 : ucv;
+      a.out[off] = rk_combine<STAGE>(uc, u0, a.dt, r);
+    }
+    u0 = u1;
+    u1 = u2;
+    u2 = u3;
+#pragma unroll
+    for (int s = 0; s < 5; s++) dz[s] = dz[s + 1];
+#pragma unroll
+    for (int s = 0; s < 3; s++) qz[s] = qz[s + 1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 stages that need both one-sided derivatives (normal motion and
+// reinitialization), on the shared quads of fused_advect_f32_kernel.  The
+// minus derivative at m and the plus derivative at m-1 are the two orders of
+// one window D(m-2..m+2): its smoothness terms (from the quads at m-1, m, m+1)
+// are formed once and each order finishes from them.  Along x and y each cell
+// forms its own window, keeps the minus derivative and hands the plus
+// derivative of the cell before it through LDS (the windows one cell past
+// the tile are formed by waves 6 and 7); along z the window of plane k+1
+// gives this plane's plus derivative and, carried, the next plane's minus.
+// ---------------------------------------------------------------------------
+struct Window {  // the part of window W(m) both orders read
+  float Q1, Q2, Q3, Q3d12, Q1d23, S6, t, C;
+};
+// d: D(m-2..m+2); quads at m-1, m, m+1
+__device__ __forceinline__ Window window(float d0, float d1, float d2, float d3, float d4, const Quad& qm, const Quad& q0,
+                                      const Quad& qp) {
+  using stencil::fastf::rcp;
+  const float b1 = fmaf(2.f, qm.A, -qm.C), b3 = fmaf(2.f, qp.A, qp.C);
+  const float im = rcp(fmaf(amax5(d0, d1, d2, d3, d4), amax5(d0, d1, d2, d3, d4), 1e-30f));
+  const float e1 = fmaf(fmaf(b1, b1, qm.T), im, 4e-6f), e2 = fmaf(fmaf(q0.C, q0.C, q0.T), im, 4e-6f),
+              e3 = fmaf(fmaf(b3, b3, qp.T), im, 4e-6f);
+  Window w;
+  w.Q1 = e1 * e1;
+  w.Q2 = e2 * e2;
+  w.Q3 = e3 * e3;
+  w.Q3d12 = w.Q3 * (qm.A - q0.A);
+  w.Q1d23 = w.Q1 * (q0.A - qp.A);
+  w.S6 = (6.f * w.Q1) * w.Q3;
+  w.t = fmaf(1.f / 12.f, q0.A, d2);
+  w.C = q0.C;
+  return w;
+}
+// minus derivative at m (undivided): weno5(D(m-2), .., D(m+2))
+__device__ __forceinline__ float win_minus(const Window& w) {
+  const float num = w.Q2 * fmaf(w.Q3d12, 1.f / 3.f, 0.5f * w.Q1d23);
+  const float den = fmaf(w.Q2, fmaf(3.f, w.Q1, w.Q3), w.S6);
+  return fmaf(num, stencil::fastf::rcp(den), fmaf(-0.25f, w.C, w.t));
+}
+// plus derivative at m-1 (undivided): weno5(D(m+2), .., D(m-2))
+__device__ __forceinline__ float win_plus(const Window& w) {
+  const float num = -(w.Q2 * fmaf(w.Q1d23, 1.f / 3.f, 0.5f * w.Q3d12));
+  const float den = fmaf(w.Q2, fmaf(3.f, w.Q3, w.Q1), w.S6);
+  return fmaf(num, stencil::fastf::rcp(den), fmaf(0.25f, w.C, w.t));
+}
+
+// Godunov upwinding of one axis from the undivided one-sided derivatives:
+// max(max(m, 0)^2, min(p, 0)^2) = max(m, -p, 0)^2 for a positive speed (or
+// for reinitialization where the sign source is positive), max(-m, p, 0)^2
+// otherwise; the 1/dx of the derivatives applied to the selected value.
+__device__ __forceinline__ float godunov_axis(bool pos, float p, float m, float inv) {
+  const float s = pos ? fmaxf(fmaxf(m, -p), 0.f) : fmaxf(fmaxf(-m, p), 0.f);
+  return s * inv;
+}
+
+template <int RHS, int STAGE, class F>
+__global__ void __launch_bounds__(TX* TY) fused_pm2_f32_kernel(FusedArgs a) {
+  __shared__ F plane[PLANE];
+  __shared__ F dxs[TY][AXD];
+  __shared__ Quad qxs[TY][AXQ];
+  __shared__ F dys[AYD][TX];
+  __shared__ Quad qys[AYQ][TX];
+  __shared__ F xplus[TY][TX];  // plus derivative along x (undivided), handed from the cell after
+  __shared__ F yplus[TY][TX];
+  static_assert(AYD - TY <= TY && AYQ - TY <= TY && TY * (AXD - TX) <= TX && TY * (AXQ - TX) <= TX && TY >= 8,
+                "one pass of waves covers the rows past the tile; waves 5 .. 7 the positions past each row");
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int nt = a.nti * a.ntj * a.ntk;
+  const int b = blockIdx.x;
+  const int per = (nt + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= nt) return;
+  const int ti = t % a.nti, tj = (t / a.nti) % a.ntj, tk = t / (a.nti * a.ntj);
+  const int i0 = a.fb.ilo + ti * TX, j0 = a.fb.jlo + tj * TY;
+  const int kb = a.fb.klo + tk * a.kc;
+  const int ke = min(kb + a.kc - 1, a.fb.khi);
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int i = i0 + lx, j = j0 + ly;
+  const Box gb = a.gb;
+  const bool active = i <= a.fb.ihi && j <= a.fb.jhi;
+  const bool own_in = i <= gb.ihi && j <= gb.jhi;
+  const long long own = (long long)(i - gb.ilo) + (long long)a.sy * (j - gb.jlo);
+  long long hoff = -1;
+  int hslot = 0;
+  if (tid < NHALO) {
+    int px, py;
+    halo_pos(tid, px, py);
+    const int gi = i0 - H + px, gj = j0 - H + py;
+    hslot = py * PX + px;
+    if (gi >= gb.ilo && gi <= gb.ihi && gj >= gb.jlo && gj <= gb.jhi)
+      hoff = (long long)(gi - gb.ilo) + (long long)a.sy * (gj - gb.jlo);
+  }
+  const F* us = a.us;
+  const long long sz = a.sz;
+  const F ix = a.ix, iy = a.iy, iz = a.iz;
+  const int hd_row = lx / (AXD - TX), hd_slot = lx % (AXD - TX);
+  const int hd = hd_slot < 2 ? hd_slot : TX + hd_slot;
+  const int hq_row = lx / (AXQ - TX), hq_slot = lx % (AXQ - TX);
+  const int hq = hq_slot == 0 ? 0 : TX + hq_slot;
+  const bool hd_lane = ly == 5 && lx < TY * (AXD - TX), hq_lane = ly == 6 && lx < TY * (AXQ - TX);
+
+  F u0, u1, u2;
+  F dz[6];
+  Quad qz[4];
+  F zminus;  // minus derivative along z at plane k (undivided)
+  {
+    F w[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) w[s] = own_in ? us[own + sz * (kb - H + s - gb.klo)] : F(0);  // k-3 .. k+2
+#pragma unroll
+    for (int s = 0; s < 5; s++) dz[s] = w[s + 1] - w[s];  // D(k-2 .. k+2)
+#pragma unroll
+    for (int s = 0; s < 3; s++) qz[s] = quad(dz[s], dz[s + 1], dz[s + 2]);  // k-1 .. k+1
+    u0 = w[3];
+    u1 = w[4];
+    u2 = w[5];
+    zminus = win_minus(window(dz[0], dz[1], dz[2], dz[3], dz[4], qz[0], qz[1], qz[2]));
+  }
+
+  for (int k = kb; k <= ke; k++) {
+    const long long zoff = sz * (k - gb.klo);
+    const F u3 = own_in ? us[own + zoff + H * sz] : F(0);
+    const F hv = hoff >= 0 ? us[hoff + zoff] : F(0);
+    const long long off = own + zoff;
+    const F ucv = STAGE != 1 && active ? a.uc[off] : F(0);
+    F c0 = F(0);
+    if (active) {
+      if (RHS == kReinit) c0 = a.use_phi0 ? a.phi0[off] : F(0);
+      else if (RHS == kNormalField) c0 = a.vnf[off];
+    }
+    __syncthreads();  // the previous plane's LDS reads are done
+    plane[(ly + H) * PX + lx + H] = u0;
+    if (tid < NHALO) plane[hslot] = hv;
+    dz[5] = u3 - u2;
+    qz[3] = quad(dz[3], dz[4], dz[5]);
+    // z: the window of plane k+1 -- this plane's plus derivative and the next one's minus
+    const Window wz = window(dz[1], dz[2], dz[3], dz[4], dz[5], qz[1], qz[2], qz[3]);
+    const F zp = win_plus(wz) * iz, zm = zminus * iz;
+    zminus = win_minus(wz);
+    __syncthreads();
+    dxs[ly][lx + 2] = plane[(ly + H) * PX + lx + H] - plane[(ly + H) * PX + lx + H - 1];
+    if (hd_lane) dxs[hd_row][hd] = plane[(hd_row + H) * PX + hd + 1] - plane[(hd_row + H) * PX + hd];
+    dys[ly][lx] = plane[(ly + 1) * PX + lx + H] - plane[ly * PX + lx + H];
+    if (ly < AYD - TY) dys[ly + TY][lx] = plane[(ly + TY + 1) * PX + lx + H] - plane[(ly + TY) * PX + lx + H];
+    __syncthreads();
+    qxs[ly][lx + 1] = quad(dxs[ly][lx + 1], dxs[ly][lx + 2], dxs[ly][lx + 3]);
+    if (hq_lane) qxs[hq_row][hq] = quad(dxs[hq_row][hq], dxs[hq_row][hq + 1], dxs[hq_row][hq + 2]);
+    qys[ly][lx] = quad(dys[ly][lx], dys[ly + 1][lx], dys[ly + 2][lx]);
+    if (ly < AYQ - TY) qys[ly + TY][lx] = quad(dys[ly + TY][lx], dys[ly + TY + 1][lx], dys[ly + TY + 2][lx]);
+    __syncthreads();
+    // x and y: this cell's windows; the plus derivative of the cell before goes through LDS
+    const F* dr = dxs[ly];
+    const Window wx = window(dr[lx], dr[lx + 1], dr[lx + 2], dr[lx + 3], dr[lx + 4], qxs[ly][lx], qxs[ly][lx + 1],
+                          qxs[ly][lx + 2]);
+    const Window wy = window(dys[ly][lx], dys[ly + 1][lx], dys[ly + 2][lx], dys[ly + 3][lx], dys[ly + 4][lx], qys[ly][lx],
+                          qys[ly + 1][lx], qys[ly + 2][lx]);
+    if (lx > 0) xplus[ly][lx - 1] = win_plus(wx);
+    if (ly > 0) yplus[ly - 1][lx] = win_plus(wy);
+    if (ly == 6 && lx < TY) {  // the window one cell past each row along x
+      const F* e = dxs[lx];
+      xplus[lx][TX - 1] = win_plus(window(e[TX], e[TX + 1], e[TX + 2], e[TX + 3], e[TX + 4], qxs[lx][TX],
+                                          qxs[lx][TX + 1], qxs[lx][TX + 2]));
+    }
+    if (ly == 7)  // the window one row past the tile along y
+      yplus[TY - 1][lx] = win_plus(window(dys[TY][lx], dys[TY + 1][lx], dys[TY + 2][lx], dys[TY + 3][lx],
+                                          dys[TY + 4][lx], qys[TY][lx], qys[TY + 1][lx], qys[TY + 2][lx]));
+    const F xm = win_minus(wx) * ix, ym = win_minus(wy) * iy;
+    __syncthreads();
+    if (active) {
+      const F xp = xplus[ly][lx] * ix, yp = yplus[ly][lx] * iy;
+      F r;
+      if (RHS == kReinit) {
+        const F s = a.use_phi0 ? c0 : u0;
+        const bool pos = s > F(0);
+        const F gx = godunov_axis(pos, xp, xm, F(1)), gy = godunov_axis(pos, yp, ym, F(1)),
+                gz = godunov_axis(pos, zp, zm, F(1));
+        const F n2 = fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+        r = F(0);
+        if (fabsf(s) >= kZeroTol) {
+          const F sgn = s * __builtin_amdgcn_rsqf(fmaf(s, s, a.use_phi0 ? a.dx_sq : n2 * a.dx_sq));
+          r = sgn * (F(1) - __builtin_amdgcn_sqrtf(n2));
+        }
+      } else {
+        const F v = RHS == kNormalField ? c0 : a.vn;
+        const bool pos = v > F(0);
+        const F gx = godunov_axis(pos, xp, xm, F(1)), gy = godunov_axis(pos, yp, ym, F(1)),
+                gz = godunov_axis(pos, zp, zm, F(1));
+        r = F(0);
+        if (fabsf(v) >= kZeroTol) r = r - v * __builtin_amdgcn_sqrtf(fmaf(gz, gz, fmaf(gy, gy, gx * gx)));
+      }
+      const F uc = STAGE == 1 ? u0 : ucv;
+      a.out[off] = rk_combine<STAGE>(uc, u0, a.dt, r);
+    }
+    u0 = u1;
+    u1 = u2;
+    u2 = u3;
+#pragma unroll
+    for (int s = 0; s < 5; s++) dz[s] = dz[s + 1];
+#pragma unroll
+    for (int s = 0; s < 3; s++) qz[s] = qz[s + 1];
+  }
+}
+
+// FP32 stage kernels: 2 = the shared-quantity kernels (fused_advect_f32_kernel,
+// fused_pm2_f32_kernel), 1 = the per-cell window kernels (fused_stage_kernel,
+// fused_pm_f32_kernel).  FP64 always runs fused_stage_kernel.
+int g_f32_kernels = 2;
+
 template <int RHS, int STAGE>
 void launch_stage(const FusedArgs& a, int nblocks) {
-  if constexpr (sizeof(real_t) == 4 && (RHS == kNormalConst || RHS == kNormalField || RHS == kReinit))
-    hipLaunchKernelGGL((fused_pm_f32_kernel<RHS, STAGE, real_t>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
-  else
+  if constexpr (sizeof(real_t) == 4 && (RHS == kNormalConst || RHS == kNormalField || RHS == kReinit)) {
+    if (g_f32_kernels == 2)
+      hipLaunchKernelGGL((fused_pm2_f32_kernel<RHS, STAGE, real_t>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+    else
+      hipLaunchKernelGGL((fused_pm_f32_kernel<RHS, STAGE, real_t>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+  } else if constexpr (sizeof(real_t) == 4 && (RHS == kAdvectField || RHS == kAdvectDeform)) {
+    if (g_f32_kernels == 2)
+      hipLaunchKernelGGL((fused_advect_f32_kernel<RHS, STAGE, real_t>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+    else
+      hipLaunchKernelGGL((fused_stage_kernel<RHS, STAGE>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
+  } else
     hipLaunchKernelGGL((fused_stage_kernel<RHS, STAGE>), dim3(nblocks), dim3(TX, TY), 0, stream(), a);
 }
 
@@ -546,3 +985,13 @@ LSMHIP_EXPORT int lsmhip_deform_velocity_fill(real_t* vx, real_t* vy, real_t* vz
 }
 
 LSMHIP_EXPORT void lsmhip_set_fused_kchunk(int kc) { g_kchunk = kc < 1 ? 1 : kc; }
+
+// FP32 build: 2 (default) selects the stage kernels that share each
+// position's WENO5 terms between neighbouring cells, 1 the kernels that form a
+// window per cell (same scheme, different rounding; for comparisons).  Returns
+// the previous choice; the FP64 build has one set of kernels and ignores it.
+LSMHIP_EXPORT int lsmhip_set_fused_f32_kernels(int which) {
+  const int prev = g_f32_kernels;
+  if (which == 1 || which == 2) g_f32_kernels = which;
+  return prev;
+}

