@@ -89,11 +89,11 @@ int lsmhip_deform_velocity_fill(LSMLIB_REAL *vel_x, LSMLIB_REAL *vel_y, LSMLIB_R
                                 LSMHIP_BOX_ARGS(b), const LSMLIB_REAL *const *tabs, LSMLIB_REAL c);
 /* k planes a fused-stage block marches through (default 64). */
 void lsmhip_set_fused_kchunk(int planes);
-/* FP32 library: 2 (default) selects the stage kernels that form each
- * position's WENO5 terms once and share them between neighbouring cells, 1 the
- * kernels that form a window per cell (the same scheme with different
- * rounding, kept for comparison).  Returns the previous choice; ignored by the
- * FP64 library. */
+/* FP32 library: 1 (default) selects the stage kernels that form a WENO5
+ * window per cell, 2 the kernels that form each position's WENO5 terms once and
+ * share them between neighbouring cells (the same scheme with different
+ * rounding; fewer instructions but more registers, and slower on gfx950; kept
+ * for comparison).  Returns the previous choice; ignored by the FP64 library. */
 int lsmhip_set_fused_f32_kernels(int which);
 
 /* ---- fused narrow-band stages: one TVD-RK stage of normal motion on a band ----

@@ -425,22 +425,26 @@ __device__ __forceinline__ float amax5(float a, float b, float c, float d, float
 }
 
 // The upwinded reconstruction (in undivided units) at position m of one axis
-// from D(m-2..m+3) (d[0..5]) and the quads at m-1..m+2 (q[0..3]): the minus
-// side for vel > 0, the plus side otherwise, 0 when |vel| < tol.  Each side
-// finishes its own reconstruction, so a wave of one sign runs one side only.
-__device__ __forceinline__ float upwind_q(float vel, float tol, const float d[6], const Quad q[4]) {
+// from D(m-2..m+3) (d[0], d[DS], .., d[5 DS]) and the quads at m-1..m+2 (q[0],
+// q[QS], .., q[3 QS]): the minus side for vel > 0, the plus side otherwise, 0
+// when |vel| < tol.  Each side reads and finishes only its own terms, so a
+// wave of one sign loads and evaluates one side.
+template <int DS, int QS, class DP, class QP>
+__device__ __forceinline__ float upwind_q(float vel, float tol, DP d, QP q) {
   float r = 0.f;
   if (!(fabsf(vel) < tol)) {
     if (vel > 0.f) {
-      const float b1 = fmaf(2.f, q[0].A, -q[0].C), b3 = fmaf(2.f, q[2].A, q[2].C);
-      r = weno_core(fmaf(b1, b1, q[0].T), fmaf(q[1].C, q[1].C, q[1].T), fmaf(b3, b3, q[2].T), q[0].A - q[1].A,
-                    q[1].A - q[2].A, fmaf(-0.25f, q[1].C, fmaf(1.f / 12.f, q[1].A, d[2])),
-                    amax5(d[0], d[1], d[2], d[3], d[4]));
+      const Quad q0 = q[0], q1 = q[QS], q2 = q[2 * QS];
+      const float b1 = fmaf(2.f, q0.A, -q0.C), b3 = fmaf(2.f, q2.A, q2.C);
+      r = weno_core(fmaf(b1, b1, q0.T), fmaf(q1.C, q1.C, q1.T), fmaf(b3, b3, q2.T), q0.A - q1.A, q1.A - q2.A,
+                    fmaf(-0.25f, q1.C, fmaf(1.f / 12.f, q1.A, d[2 * DS])),
+                    amax5(d[0], d[DS], d[2 * DS], d[3 * DS], d[4 * DS]));
     } else {
-      const float b1 = fmaf(2.f, q[3].A, q[3].C), b3 = fmaf(2.f, q[1].A, -q[1].C);
-      r = weno_core(fmaf(b1, b1, q[3].T), fmaf(q[2].C, q[2].C, q[2].T), fmaf(b3, b3, q[1].T), q[3].A - q[2].A,
-                    q[2].A - q[1].A, fmaf(0.25f, q[2].C, fmaf(1.f / 12.f, q[2].A, d[3])),
-                    amax5(d[1], d[2], d[3], d[4], d[5]));
+      const Quad q1 = q[QS], q2 = q[2 * QS], q3 = q[3 * QS];
+      const float b1 = fmaf(2.f, q3.A, q3.C), b3 = fmaf(2.f, q1.A, -q1.C);
+      r = weno_core(fmaf(b1, b1, q3.T), fmaf(q2.C, q2.C, q2.T), fmaf(b3, b3, q1.T), q3.A - q2.A, q2.A - q1.A,
+                    fmaf(0.25f, q2.C, fmaf(1.f / 12.f, q2.A, d[3 * DS])),
+                    amax5(d[DS], d[2 * DS], d[3 * DS], d[4 * DS], d[5 * DS]));
     }
   }
   return r;
@@ -560,21 +564,9 @@ __global__ void __launch_bounds__(TX* TY) fused_advect_f32_kernel(FusedArgs a) {
         vy = c1 * a.iy;
         wz = c2 * a.iz;
       }
-      F dx6[6], dy6[6];
-      Quad qx4[4], qy4[4];
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        dx6[s] = dxs[ly][lx + s];
-        dy6[s] = dys[ly + s][lx];
-      }
-#pragma unroll
-      for (int s = 0; s < 4; s++) {
-        qx4[s] = qxs[ly][lx + s];
-        qy4[s] = qys[ly + s][lx];
-      }
-      const F px = upwind_q(ux, tolx, dx6, qx4);
-      const F py = upwind_q(vy, toly, dy6, qy4);
-      const F pz = upwind_q(wz, tolz, dz, qz);
+      const F px = upwind_q<1, 1>(ux, tolx, &dxs[ly][lx], &qxs[ly][lx]);
+      const F py = upwind_q<TX, TX>(vy, toly, &dys[ly][lx], &qys[ly][lx]);
+      const F pz = upwind_q<1, 1>(wz, tolz, dz, qz);
       // r = -(u phi_x + v phi_y + w phi_z), the velocities already divided by the spacing
       const F r = F(0) - (ux * px + vy * py + wz * pz);
       const F uc = STAGE == 1 ? u0 : ucv;
@@ -793,10 +785,15 @@ __global__ void __launch_bounds__(TX* TY) fused_pm2_f32_kernel(FusedArgs a) {
   }
 }
 
-// FP32 stage kernels: 2 = the shared-quantity kernels (fused_advect_f32_kernel,
-// fused_pm2_f32_kernel), 1 = the per-cell window kernels (fused_stage_kernel,
-// fused_pm_f32_kernel).  FP64 always runs fused_stage_kernel.
-int g_f32_kernels = 2;
+// FP32 stage kernels: 1 (default) = the per-cell window kernels
+// (fused_stage_kernel, fused_pm_f32_kernel), 2 = the shared-quantity kernels
+// (fused_advect_f32_kernel, fused_pm2_f32_kernel).  The second set issues
+// 14-20 % fewer VALU instructions per cell but holds 72-80 VGPRs and meets
+// four barriers per plane; at the 6 waves per SIMD that allows it runs slower
+// on gfx950 (0.91 vs 0.76 ms per 512^3 advection stage).  It stays as an
+// independent rounding order for precision comparisons.  FP64 always runs
+// fused_stage_kernel.
+int g_f32_kernels = 1;
 
 template <int RHS, int STAGE>
 void launch_stage(const FusedArgs& a, int nblocks) {
@@ -986,10 +983,11 @@ LSMHIP_EXPORT int lsmhip_deform_velocity_fill(real_t* vx, real_t* vy, real_t* vz
 
 LSMHIP_EXPORT void lsmhip_set_fused_kchunk(int kc) { g_kchunk = kc < 1 ? 1 : kc; }
 
-// FP32 build: 2 (default) selects the stage kernels that share each
-// position's WENO5 terms between neighbouring cells, 1 the kernels that form a
-// window per cell (same scheme, different rounding; for comparisons).  Returns
-// the previous choice; the FP64 build has one set of kernels and ignores it.
+// FP32 build: 1 (default) selects the stage kernels that form a WENO5 window
+// per cell, 2 the kernels that share each position's WENO5 terms between
+// neighbouring cells (same scheme, different rounding; slower on gfx950, kept
+// for comparisons).  Returns the previous choice; the FP64 build has one set of
+// kernels and ignores it.
 LSMHIP_EXPORT int lsmhip_set_fused_f32_kernels(int which) {
   const int prev = g_f32_kernels;
   if (which == 1 || which == 2) g_f32_kernels = which;

@@ -109,13 +109,20 @@ def test_fused_rk3_step_fp64_equals_dropin_chain(solvers, kind):
     assert np.array_equal(ha, hb), np.abs(ha - hb).max()
 
 
+@pytest.mark.parametrize("kernels", [1, 2])
 @pytest.mark.parametrize("kind", ["normal", "advect", "reinit"])
-def test_fused_fp32_close_to_fp64(solvers, kind):
+def test_fused_fp32_close_to_fp64(solvers, kind, kernels):
+    """Both FP32 kernel sets (1: a WENO5 window per cell, the default; 2: each
+    position's terms shared between cells) within 2e-5 of FP64 after five
+    steps."""
+    import ctypes
     g, s64, s32 = solvers
     from lsmlib_amd.solver import phi_sphere
     phi_np = phi_sphere(g, (0.5, 0.45, 0.42), 0.3)
     if kind == "reinit":
         phi_np = 1.7 * phi_np  # |grad phi| = 1.7: the reinitialization RHS is O(1)
+    select = s32.lib.fn("lsmhip_set_fused_f32_kernels", [ctypes.c_int])
+    prev = select(kernels)
     res = []
     for s in (s64, s32):
         a = s.field(phi_np)
@@ -130,5 +137,6 @@ def test_fused_fp32_close_to_fp64(solvers, kind):
                 s.rk3_fused(a, w, s.stage_deform, 0.4 * g.dx[0], c=1.0)
         s.lib.check()
         res.append(g.interior(s.host(a)))
+    assert select(prev) == kernels
     # FP32 storage and arithmetic: relative-to-O(1) errors of a few ulp per step
     assert np.abs(res[0] - res[1]).max() < 2e-5

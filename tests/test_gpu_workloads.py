@@ -212,16 +212,16 @@ def test_c4_precision_study_to_t3_at_128():
 
     At t = 0.25 (159 steps) FP32 tracks FP64 to rounding: relative volume of
     {phi < 0} (FP64 compensated sum) within 1e-7, L-inf within 6 dx of the
-    interface below 2e-6, L1 over all cells below 1e-5 (measured 8e-9, 4e-7,
-    1.7e-6).  By t = 3 the flow has stretched the sphere into a sheet a few
-    cells thick and back; the 128^3 grid under-resolves it (FP64 itself ends
-    26 % short of the initial volume), and rounding differences grow with the
-    stretching: the bounds there are volume 1e-3, band L-inf dx/4, L1 1e-4
-    (measured 2.2e-4, 7.2e-4 = 0.09 dx, 1.8e-5; the previous FP32 stage
-    kernels, which share no arithmetic order with these, measured 9.8e-5,
-    2.9e-4, 2.1e-5 against the same FP64 run).  FP32's error against the
-    initial sphere equals FP64's to within 1e-3 of the volume and dx/4 near
-    the interface: rounding does not change the discretization error."""
+    interface below 2e-6, L1 over all cells below 1e-5 (measured 1.3e-8,
+    3.8e-7, 1.7e-6).  By t = 3 the flow has stretched the sphere into a sheet
+    a few cells thick and back; the 128^3 grid under-resolves it (FP64 itself
+    ends 26 % short of the initial volume), and rounding differences grow with
+    the stretching: the bounds there are volume 1e-3, band L-inf dx/4, L1 1e-4
+    (measured 9.8e-5, 2.9e-4 = 0.04 dx, 2.1e-5; the shared-term FP32 kernels,
+    another rounding order of the same scheme, measured 2.2e-4, 7.2e-4,
+    1.8e-5 against the same FP64 run).  FP32's error against the initial
+    sphere equals FP64's to within 1e-3 of the volume and dx/4 near the
+    interface: rounding does not change the discretization error."""
     from lsmlib_amd.solver import precision_study
     h = 1.0 / 128
     bounds = {0.25: (1e-7, 2e-6, 1e-5), 3.0: (1e-3, h / 4, 1e-4)}
