@@ -188,6 +188,20 @@ int lsmhip_heaviside_mismatch(double *loss, LSMLIB_REAL *grad, const LSMLIB_REAL
 int lsmhip_volume_phi_lt0_f64(double *volume, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(phi_gb), LSMHIP_BOX_ARGS(ib),
                               const double *dx, const double *dy, const double *dz, const double *epsilon);
 
+/* ---- allocation-free variant ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): lsm3dvolumeregionphilessthanzero_ (the same kernels and block
+ * count, so the same result bit for bit) with its per-block partial sums and
+ * result in a file-scope device array of 64 KiB that each GPU holds until the
+ * process ends, instead of library workspace.  It calls no hipMalloc or
+ * hipFree.  Boxes of 2^31 cells or more are refused before anything is
+ * launched (LSMHIP_ERR_SIZE).  Calls that overlap on one GPU (several host
+ * threads or streams) race on that array.  Returns 0 or -1. */
+int lsmhip_volume_region_phi_lt0_noalloc(LSMLIB_REAL *volume, const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(phi_gb),
+                                         LSMHIP_BOX_ARGS(ib), const LSMLIB_REAL *dx, const LSMLIB_REAL *dy,
+                                         const LSMLIB_REAL *dz, const LSMLIB_REAL *epsilon);
+
 LSMHIP_END_DECLS
 
 #endif

@@ -158,10 +158,12 @@ bool box_max(const char* who, const Box& b, const Op& op, double* result) {
   return true;
 }
 
+// buf: caller-provided device memory of at least 2 * kMaxBlocks + 2 doubles
+// for the partials and the result, or null for the library workspace.
 template <class Op>
-bool box_sum(const char* who, const Box& b, const Op& op, double* result) {
+bool box_sum(const char* who, const Box& b, const Op& op, double* result, double* buf = nullptr) {
   const int nb = blocks_for(b);
-  double* ws = (double*)workspace(sizeof(double) * (2 * nb + 2));
+  double* ws = buf ? buf : (double*)workspace(sizeof(double) * (2 * nb + 2));
   if (!ws) return false;
   hipLaunchKernelGGL(sum_kernel<Op>, dim3(nb), dim3(kThreads), 0, stream(), b, op, ws);
   hipLaunchKernelGGL(sum_finish_kernel, dim3(1), dim3(kThreads), 0, stream(), ws, nb, ws + 2 * nb);

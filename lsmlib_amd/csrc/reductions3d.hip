@@ -8,6 +8,8 @@
 // computes it and the terms are summed in FP64 with a fixed-order
 // compensated (Neumaier) reduction: deterministic, and at least as accurate.
 // Both precisions accumulate in FP64.
+#include <mutex>
+
 #include "lsmhip_core.hpp"
 #include "reduce.hpp"
 #include "stencil.hpp"
@@ -161,8 +163,11 @@ LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldt_(real_t* dt, const real_t*
 }
 
 // ----- integrals -----------------------------------------------------------
+// buf: device memory for the sum's partials (2 * reduce::kMaxBlocks + 2
+// doubles), or null for the library workspace
 static double volume(const char* who, const real_t* F, const Box& fg, const real_t* phi, const Box& pg,
-                     const Box& ib, real_t dx, real_t dy, real_t dz, real_t eps, int sign, bool* good) {
+                     const Box& ib, real_t dx, real_t dy, real_t dz, real_t eps, int sign, bool* good,
+                     double* buf = nullptr) {
   *good = false;
   if (ib.empty()) {
     *good = true;
@@ -173,7 +178,7 @@ static double volume(const char* who, const real_t* F, const Box& fg, const real
   VolumeTerm op{view(phi, pg), F ? view(F, fg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, eps, dx * dy * dz,
                 sign};
   double s = 0;
-  *good = reduce::box_sum(who, ib, op, &s);
+  *good = reduce::box_sum(who, ib, op, &s, buf);
   return s;
 }
 
@@ -270,9 +275,73 @@ LSMHIP_EXPORT int lsmhip_volume_phi_lt0_f64(double* out, const real_t* phi, LSMH
   return good ? 0 : -1;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free volume of {phi < 0} (no counterpart in the reference): the
+// computation of lsm3dvolumeregionphilessthanzero_ -- the same kernels and
+// block count, hence the same bits -- with the per-block partial sums and the
+// result kept in `scratch`, a file-scope device array, instead of the
+// library's workspace, which may call hipMalloc.  For callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops).
+//
+// Each GPU holds its own copy of scratch (2 * reduce::kMaxBlocks + 2 doubles,
+// 64 KiB) until the process ends.  The sum kernel never runs more than
+// reduce::kMaxBlocks blocks, so scratch covers every box the library takes
+// (fewer than 2^31 cells); a larger box is refused before anything is
+// launched.  Every slot the finishing kernel reads is written earlier in the
+// same call, so nothing needs clearing between calls; calls that overlap on
+// one GPU (several host threads or streams) race on scratch.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kScratchDoubles = 2 * reduce::kMaxBlocks + 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxScratchDevices] = {};
+
+// scratch's device address on the current device, looked up on the first call
+// made with that device current (host code must not use the name itself)
+double* scratch_address(const char* who) {
+  int d = 0;
+  if (!check_hip(hipGetDevice(&d), who)) return nullptr;
+  if (d < 0 || d >= kMaxScratchDevices) {
+    set_error(kErrArg, "%s: device %d beyond the %d this entry point serves", who, d, kMaxScratchDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[d]) {
+    void* p = nullptr;
+    if (!check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), who)) return nullptr;
+    g_scratch[d] = static_cast<double*>(p);
+  }
+  return g_scratch[d];
+}
+}  // namespace
+
+LSMHIP_EXPORT int lsmhip_volume_region_phi_lt0_noalloc(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
+                                                       LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                                       const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS_OR(-1, phi);
+  const char* who = "lsmhip_volume_region_phi_lt0_noalloc";
+  if (!out) {
+    set_error(kErrArg, "%s: out is NULL", who);
+    return -1;
+  }
+  const Box pgb = LSMHIP_BOX(pg), ibb = LSMHIP_BOX(ib);
+  // the sizes are checked before any device work (volume() checks them again)
+  if (!ok(who, ibb, {{pgb, "phi"}})) return -1;
+  static_assert(kScratchDoubles >= 2 * reduce::kMaxBlocks + 2, "scratch holds every block's partial pair");
+  double* buf = scratch_address(who);
+  if (!buf) return -1;
+  bool good;
+  Box none{0, -1, 0, -1, 0, -1};
+  double s = volume(who, nullptr, none, phi, pgb, ibb, *dx, *dy, *dz, *eps, -1, &good, buf);
+  if (good) *out = real_t(s);
+  return good ? 0 : -1;
+}
+
 // Smoothed Heaviside and delta (src/toolbox/lsm_calculus_toolbox.f:25-92); host scalars.
 LSMHIP_EXPORT void lsmheaviside_(real_t* H, const real_t* x, const real_t* epsilon) {
-
   const real_t e = *epsilon, v = *x;
   if (v < -e) *H = real_t(0);
   else if (v > e) *H = real_t(1);
@@ -282,7 +351,6 @@ LSMHIP_EXPORT void lsmheaviside_(real_t* H, const real_t* x, const real_t* epsil
   }
 }
 LSMHIP_EXPORT void lsmdeltafunction_(real_t* delta, const real_t* x, const real_t* epsilon) {
-
   const real_t e = *epsilon, v = *x;
   if (fabs(v) > e) *delta = real_t(0);
   else {

@@ -38,17 +38,19 @@ def volume_case(seed=0):
     return phi, grads
 
 
-def integral_terms(which, phi, grads):
+def integral_terms(which, phi, grads=None, gbox=GB, ibox=IB, h=H_, eps=EPS):
     """The oracle's addends of the volume (which 0: phi < 0, 1: phi > 0) or area
-    (2) sum over IB, in loop order."""
+    (2) sum over ibox, in loop order (the volumes read no gradients)."""
     lib = ctypes.CDLL(H.ORACLE_SO)
     f = lib.oracle_integral_terms
     f.restype = None
-    n = (IB[1] - IB[0] + 1) * (IB[3] - IB[2] + 1) * (IB[5] - IB[4] + 1)
+    n = (ibox[1] - ibox[0] + 1) * (ibox[3] - ibox[2] + 1) * (ibox[5] - ibox[4] + 1)
     out = np.empty(n)
-    k1, gb = _ints(*GB)
-    k2, ib = _ints(*IB)
-    d = [ctypes.c_double(v) for v in (H_, H_, H_, EPS)]
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    grads = grads if grads is not None else [phi] * 3
+    k1, gb = _ints(*gbox)
+    k2, ib = _ints(*ibox)
+    d = [ctypes.c_double(v) for v in (h, h, h, eps)]
     f(_ptr(out), ctypes.c_int(which), _ptr(phi), *gb, _ptr(grads[0]), _ptr(grads[1]), _ptr(grads[2]), *gb, *ib,
       *[ctypes.byref(x) for x in d])
     return out
