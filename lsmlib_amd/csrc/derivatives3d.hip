@@ -11,6 +11,7 @@
 // neither read nor written.  Every difference a fill-box cell uses lies inside
 // the range the reference computes, so its 1e10 fill values never enter.
 #include "lsmhip_core.hpp"
+#include "march.hpp"
 #include "stencil.hpp"
 
 using namespace lsmhip;
@@ -79,38 +80,33 @@ __global__ void __launch_bounds__(256) hj_upwind_kernel(Box fb, Axes3 out, View<
                                      : stencil::upwind<ORDER>(f[c][d], inv[d], vel[c][d] > real_t(0));
 }
 
-__global__ void __launch_bounds__(256) central2_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
-                                                       real_t fy, real_t fz) {
-  LSMHIP_CELL_PAIR(fb);
-  // loads of both cells first: a load after a store to a possibly aliasing
-  // output would wait for it
-  real_t xp[2], xm[2], yp[2], ym[2], zp[2], zm[2];
-#pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk) {
-      const int k = k0 + c;
-      xp[c] = phi(i + 1, j, k);
-      xm[c] = phi(i - 1, j, k);
-      yp[c] = phi(i, j + 1, k);
-      ym[c] = phi(i, j - 1, k);
-      zp[c] = phi(i, j, k + 1);
-      zm[c] = phi(i, j, k - 1);
-    }
-#pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk) {
-      const int k = k0 + c;
-      out.o[0](i, j, k) = (xp[c] - xm[c]) * fx;
-      out.o[1](i, j, k) = (yp[c] - ym[c]) * fy;
-      out.o[2](i, j, k) = (zp[c] - zm[c]) * fz;
-    }
-}
+// Second-order central gradient and Laplacian on the plane march
+// (src/toolbox/lsm_spatial_derivatives3d.f.in, lsm3dCentralGradOrder2 and
+// lsm3dLaplacianOrder2), in the reference's operation order.
+struct Central2Star {
+  Axes3 out;
+  real_t fx, fy, fz;
+  __device__ void operator()(int i, int j, int k, real_t, real_t xm, real_t xp, real_t ym, real_t yp, real_t zm,
+                             real_t zp) const {
+    out.o[0](i, j, k) = (xp - xm) * fx;
+    out.o[1](i, j, k) = (yp - ym) * fy;
+    out.o[2](i, j, k) = (zp - zm) * fz;
+  }
+};
+struct Laplacian2Star {
+  View<real_t> out;
+  real_t ax, ay, az;
+  __device__ void operator()(int i, int j, int k, real_t c, real_t xm, real_t xp, real_t ym, real_t yp, real_t zm,
+                             real_t zp) const {
+    out(i, j, k) = ax * (xp - real_t(2) * c + xm) + ay * (yp - real_t(2) * c + ym) + az * (zp - real_t(2) * c + zm);
+  }
+};
 
 __global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
                                                        real_t fy, real_t fz) {
   LSMHIP_CELL(fb);
   const real_t e = real_t(8);
-  real_t f[3][7];  // loads first, as in central2_kernel
+  real_t f[3][7];  // loads first: a load after a store to a possibly aliasing output would wait for it
 #pragma unroll
   for (int d = 0; d < 3; d++) load_line<2>(phi, i, j, k, d, f[d]);
   const real_t r[3] = {fx, fy, fz};
@@ -118,33 +114,64 @@ __global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<c
   for (int d = 0; d < 3; d++) out.o[d](i, j, k) = (-f[d][5] + e * f[d][4] + f[d][1] - e * f[d][2]) * r[d];
 }
 
-__global__ void __launch_bounds__(256) laplacian2_kernel(Box fb, View<real_t> out, View<const real_t> phi,
-                                                         real_t ax, real_t ay, real_t az) {
-  LSMHIP_CELL_PAIR(fb);
-  real_t v[2][7];  // centre, x+, x-, y+, y-, z+, z- of both cells, loaded first
+// lsm3dCentralHessian (src/toolbox/lsm_spatial_derivatives3d.f.in:3658-3715):
+// compact second differences over dx*dx etc., mixed ones as 0.25 (f(+,+) -
+// f(-,+) - f(+,-) + f(-,-)) / (dx*dy), in that operation order; the 19 values
+// of the cell loaded before any store.
+struct Hessian6 {
+  View<real_t> xx, xy, xz, yy, yz, zz;
+};
+__global__ void __launch_bounds__(256) central_hessian_kernel(Box fb, Hessian6 out, View<const real_t> phi,
+                                                              real_t dxx, real_t dxy, real_t dxz, real_t dyy,
+                                                              real_t dyz, real_t dzz) {
+  LSMHIP_CELL(fb);
+  const real_t c = phi(i, j, k);
+  const real_t xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, j + 1, k), ym = phi(i, j - 1, k),
+               zp = phi(i, j, k + 1), zm = phi(i, j, k - 1);
+  const real_t xy[4] = {phi(i + 1, j + 1, k), phi(i - 1, j + 1, k), phi(i + 1, j - 1, k), phi(i - 1, j - 1, k)};
+  const real_t xz[4] = {phi(i + 1, j, k + 1), phi(i - 1, j, k + 1), phi(i + 1, j, k - 1), phi(i - 1, j, k - 1)};
+  const real_t yz[4] = {phi(i, j + 1, k + 1), phi(i, j - 1, k + 1), phi(i, j + 1, k - 1), phi(i, j - 1, k - 1)};
+  const real_t q = real_t(0.25);
+  out.xx(i, j, k) = (xp - real_t(2) * c + xm) / dxx;
+  out.xy(i, j, k) = q * (xy[0] - xy[1] - xy[2] + xy[3]) / dxy;
+  out.xz(i, j, k) = q * (xz[0] - xz[1] - xz[2] + xz[3]) / dxz;
+  out.yy(i, j, k) = (yp - real_t(2) * c + ym) / dyy;
+  out.yz(i, j, k) = q * (yz[0] - yz[1] - yz[2] + yz[3]) / dyz;
+  out.zz(i, j, k) = (zp - real_t(2) * c + zm) / dzz;
+}
+
+// One axis of lsm3dPhiUpwindGradF (src/toolbox/lsm_spatial_derivatives3d.f.in:
+// 3270-3458): the one-sided derivative from the neighbour whose phi is nearer
+// zero (smaller for phi > 0, larger for phi < 0), the plus one where the minus
+// neighbour does not qualify, 0 where neither does.
+__device__ __forceinline__ real_t phi_upwind_pick(real_t c, real_t nm, real_t np, real_t fp, real_t fm) {
+  if (c > real_t(0)) {
+    if (nm <= c) return np < nm ? fp : fm;
+    return np <= c ? fp : real_t(0);
+  }
+  if (nm >= c) return np > nm ? fp : fm;
+  return np >= c ? fp : real_t(0);
+}
+__global__ void __launch_bounds__(256) phi_upwind_grad_kernel(Box fb, Axes3 out, Axes3 plus, Axes3 minus,
+                                                              View<const real_t> phi) {
+  LSMHIP_CELL(fb);
+  const real_t c = phi(i, j, k);
+  if (!(c > real_t(0)) && !(c < real_t(0))) return;  // phi == 0 (or NaN): the outputs are left as they are
+  const real_t n[6] = {phi(i - 1, j, k), phi(i + 1, j, k), phi(i, j - 1, k),
+                       phi(i, j + 1, k), phi(i, j, k - 1), phi(i, j, k + 1)};
+  real_t fp[3], fm[3];
 #pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk) {
-      const int k = k0 + c;
-      v[c][0] = phi(i, j, k);
-      v[c][1] = phi(i + 1, j, k);
-      v[c][2] = phi(i - 1, j, k);
-      v[c][3] = phi(i, j + 1, k);
-      v[c][4] = phi(i, j - 1, k);
-      v[c][5] = phi(i, j, k + 1);
-      v[c][6] = phi(i, j, k - 1);
-    }
+  for (int d = 0; d < 3; d++) {
+    fp[d] = plus.o[d](i, j, k);
+    fm[d] = minus.o[d](i, j, k);
+  }
 #pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk)
-      out(i, j, k0 + c) = ax * (v[c][1] - real_t(2) * v[c][0] + v[c][2]) +
-                          ay * (v[c][3] - real_t(2) * v[c][0] + v[c][4]) +
-                          az * (v[c][5] - real_t(2) * v[c][0] + v[c][6]);
+  for (int d = 0; d < 3; d++) out.o[d](i, j, k) = phi_upwind_pick(c, n[2 * d], n[2 * d + 1], fp[d], fm[d]);
 }
 
 __global__ void __launch_bounds__(256) average_grad_kernel(Box fb, Axes3 out, Axes3 plus, Axes3 minus) {
   LSMHIP_CELL(fb);
-  real_t p[3], m[3];  // loads first, as in central2_kernel
+  real_t p[3], m[3];  // loads first, as in central4_kernel
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     p[d] = plus.o[d](i, j, k);
@@ -267,8 +294,10 @@ LSMHIP_EXPORT void lsm3dcentralgradorder2_(real_t* px, real_t* py, real_t* pz, L
   if (!check_box(g, fb, who, "grad_phi") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") || !check_size(g, who) ||
       !check_size(pg, who))
     return;
-  hipLaunchKernelGGL(central2_kernel, grid_for_pair(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g), view(phi, pg),
-                     real_t(0.5) / *dx, real_t(0.5) / *dy, real_t(0.5) / *dz);
+  const real_t fx = real_t(0.5) / *dx, fy = real_t(0.5) / *dy, fz = real_t(0.5) / *dz;
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL(march::star_march_kernel<Central2Star>, march::grid_for(tl), march::block(), 0, stream(), fb,
+                     view(phi, pg), tl, Central2Star{axes(px, py, pz, g), fx, fy, fz});
   finish(who);
 }
 
@@ -297,8 +326,43 @@ LSMHIP_EXPORT void lsm3dlaplacianorder2_(real_t* lap, LSMHIP_BOXP(lg), const rea
   if (!check_box(lg, fb, who, "laplacian_phi") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") ||
       !check_size(lg, who) || !check_size(pg, who))
     return;
-  hipLaunchKernelGGL(laplacian2_kernel, grid_for_pair(fb), block2d(), 0, stream(), fb, view(lap, lg), view(phi, pg),
-                     real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz);
+  const real_t ax = real_t(1.0) / *dx / *dx, ay = real_t(1.0) / *dy / *dy, az = real_t(1.0) / *dz / *dz;
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL(march::star_march_kernel<Laplacian2Star>, march::grid_for(tl), march::block(), 0, stream(), fb,
+                     view(phi, pg), tl, Laplacian2Star{view(lap, lg), ax, ay, az});
+  finish(who);
+}
+
+LSMHIP_EXPORT void lsm3dcentralhessian_(real_t* pxx, real_t* pxy, real_t* pxz, real_t* pyy, real_t* pyz, real_t* pzz,
+                                        LSMHIP_BOXP(hg), const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(fb),
+                                        const real_t* dx, const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(pxx, pxy, pxz, pyy, pyz, pzz, phi);
+  const char* who = "lsm3dcentralhessian_";
+  const Box fb = LSMHIP_BOX(fb), hg = LSMHIP_BOX(hg), pg = LSMHIP_BOX(pg);
+  if (fb.empty()) return;
+  if (!check_box(hg, fb, who, "hessian") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") || !check_size(hg, who) ||
+      !check_size(pg, who))
+    return;
+  hipLaunchKernelGGL(central_hessian_kernel, grid_for(fb), block2d(), 0, stream(), fb,
+                     Hessian6{view(pxx, hg), view(pxy, hg), view(pxz, hg), view(pyy, hg), view(pyz, hg),
+                              view(pzz, hg)},
+                     view(phi, pg), *dx * *dx, *dx * *dy, *dx * *dz, *dy * *dy, *dy * *dz, *dz * *dz);
+  finish(who);
+}
+
+LSMHIP_EXPORT void lsm3dphiupwindgradf_(real_t* fx, real_t* fy, real_t* fz, LSMHIP_BOXP(g), real_t* fxp, real_t* fyp,
+                                        real_t* fzp, LSMHIP_BOXP(gp), real_t* fxm, real_t* fym, real_t* fzm,
+                                        LSMHIP_BOXP(gm), const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(fx, fy, fz, fxp, fyp, fzp, fxm, fym, fzm, phi);
+  const char* who = "lsm3dphiupwindgradf_";
+  const Box fb = LSMHIP_BOX(fb), g = LSMHIP_BOX(g), gp = LSMHIP_BOX(gp), gm = LSMHIP_BOX(gm), pg = LSMHIP_BOX(pg);
+  if (fb.empty()) return;
+  if (!check_box(g, fb, who, "grad_F") || !check_box(gp, fb, who, "grad_F_plus") ||
+      !check_box(gm, fb, who, "grad_F_minus") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") ||
+      !check_size(g, who) || !check_size(gp, who) || !check_size(gm, who) || !check_size(pg, who))
+    return;
+  hipLaunchKernelGGL(phi_upwind_grad_kernel, grid_for(fb), block2d(), 0, stream(), fb, axes(fx, fy, fz, g),
+                     axes(fxp, fyp, fzp, gp), axes(fxm, fym, fzm, gm), view(phi, pg));
   finish(who);
 }
 
