@@ -26,6 +26,8 @@
 #define LSM3D_LAPLACIAN_ORDER2 lsm3dlaplacianorder2_
 #define LSM3D_AVERAGE_GRAD_PHI lsm3daveragegradphi_
 #define LSM3D_GRADIENT_MAGNITUDE lsm3dgradientmagnitude_
+#define LSM3D_PHI_UPWIND_GRAD_F lsm3dphiupwindgradf_
+#define LSM3D_CENTRAL_HESSIAN lsm3dcentralhessian_
 
 LSMHIP_BEGIN_DECLS
 
@@ -82,6 +84,17 @@ void LSM3D_AVERAGE_GRAD_PHI(LSMLIB_REAL *phi_x, LSMLIB_REAL *phi_y, LSMLIB_REAL 
 /* replaces :940 (thresholds 1e-8 on each squared component and on the result) */
 void LSM3D_GRADIENT_MAGNITUDE(const LSMLIB_REAL *phi_x, const LSMLIB_REAL *phi_y, const LSMLIB_REAL *phi_z,
                               LSMLIB_REAL *grad_phi_mag, LSMHIP_BOX_ARGS(grad_phi_gb), LSMHIP_BOX_ARGS(grad_phi_fb));
+/* replaces :832.  Cells where phi is exactly zero (or NaN) are left as they
+ * are, as in the reference. */
+void LSM3D_PHI_UPWIND_GRAD_F(LSMLIB_REAL *F_x, LSMLIB_REAL *F_y, LSMLIB_REAL *F_z, LSMHIP_BOX_ARGS(grad_F_gb),
+                             LSMLIB_REAL *F_x_plus, LSMLIB_REAL *F_y_plus, LSMLIB_REAL *F_z_plus,
+                             LSMHIP_BOX_ARGS(grad_F_plus_gb), LSMLIB_REAL *F_x_minus, LSMLIB_REAL *F_y_minus,
+                             LSMLIB_REAL *F_z_minus, LSMHIP_BOX_ARGS(grad_F_minus_gb), const LSMLIB_REAL *phi,
+                             LSMHIP_BOX_ARGS(phi_gb), LSMHIP_BOX_ARGS(fb));
+/* replaces :971 */
+void LSM3D_CENTRAL_HESSIAN(LSMLIB_REAL *phi_xx, LSMLIB_REAL *phi_xy, LSMLIB_REAL *phi_xz, LSMLIB_REAL *phi_yy,
+                           LSMLIB_REAL *phi_yz, LSMLIB_REAL *phi_zz, LSMHIP_BOX_ARGS(hessian_gb),
+                           const LSMLIB_REAL *phi, LSMHIP_BOX_ARGS(gb), LSMHIP_BOX_ARGS(fb), LSMHIP_DXYZ);
 
 LSMHIP_END_DECLS
 

@@ -83,6 +83,36 @@ _RAW = {
     "lsm3dtvdrk3stage3_": "R:un B:un R:u2 B:u2 R:uc B:uc R:rhs B:rhs B:fb S:dt",
     # --- reductions ---------------------------------------------------------------
     "lsm3dmaxnormdiff_": "SO:out R:f1 B:f1 R:f2 B:f2 B:ib",
+    "lsm3daveabsdiff_": "SO:out R:f1 B:f1 R:f2 B:f2 B:ib",
+    "lsm3dvoxelcountgreaterthanzero_": "IO:count R:phi B:phi B:fb",
+    "lsm3dvoxelcountlessthanzero_": "IO:count R:phi B:phi B:fb",
+    "lsm3dmaxnormdiffcontrolvolume_": "SO:out R:f1 B:f1 R:f2 B:f2 R:cv B:cv I:cvs B:ib",
+    "lsm3dcomputestableadvectiondtcontrolvolume_": "SO:dt R:vx R:vy R:vz B:vel R:cv B:cv I:cvs B:ib S:dx S:dy "
+                                                   "S:dz S:cfl",
+    "lsm3dcomputestablenormalveldtcontrolvolume_": "SO:dt R:vn B:vel R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm "
+                                                   "B:gm R:cv B:cv I:cvs B:ib S:dx S:dy S:dz S:cfl",
+    "lsm3dcomputestableconstnormalveldtcontrolvolume_": "SO:dt S:vn R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm "
+                                                        "B:gm R:cv B:cv I:cvs B:ib S:dx S:dy S:dz S:cfl",
+    "lsm3dvolumeintegralphilessthanzerocontrolvolume_": "SO:int R:F B:F R:phi B:phi R:cv B:cv I:cvs B:ib S:dx "
+                                                        "S:dy S:dz S:eps",
+    "lsm3dvolumeintegralphigreaterthanzerocontrolvolume_": "SO:int R:F B:F R:phi B:phi R:cv B:cv I:cvs B:ib S:dx "
+                                                           "S:dy S:dz S:eps",
+    "lsm3dsurfaceintegralcontrolvolume_": "SO:int R:F B:F R:phi B:phi R:px R:py R:pz B:g R:cv B:cv I:cvs B:ib "
+                                          "S:dx S:dy S:dz S:eps",
+    "lsm3dvoxelcountgreaterthanzerocontrolvolume_": "IO:count R:phi B:phi R:cv B:cv I:cvs B:fb",
+    "lsm3dvoxelcountlessthanzerocontrolvolume_": "IO:count R:phi B:phi R:cv B:cv I:cvs B:fb",
+    "lsm3dsurfaceareazerolevelsetdelta_": "SO:area R:dphi B:phi R:px R:py R:pz B:g B:ib S:dx S:dy S:dz",
+    "lsm3dvolumeregionphilessthanzerocontrolvolume_": "SO:vol R:phi B:phi R:cv B:cv I:cvs B:ib S:dx S:dy S:dz "
+                                                      "S:eps",
+    "lsm3dvolumeregionphigreaterthanzerocontrolvolume_": "SO:vol R:phi B:phi R:cv B:cv I:cvs B:ib S:dx S:dy "
+                                                         "S:dz S:eps",
+    "lsm3dsurfaceareazerolevelsetcontrolvolume_": "SO:area R:phi B:phi R:px R:py R:pz B:g R:cv B:cv I:cvs B:ib "
+                                                  "S:dx S:dy S:dz S:eps",
+    "lsm3dsurfaceareazerolevelsetdeltacontrolvolume_": "SO:area R:dphi B:phi R:px R:py R:pz B:g R:cv B:cv I:cvs "
+                                                       "B:ib S:dx S:dy S:dz",
+    "lsm3dphiupwindgradf_": "R:Fx R:Fy R:Fz B:g R:Fxp R:Fyp R:Fzp B:gp R:Fxm R:Fym R:Fzm B:gm R:phi B:phi B:fb",
+    "lsm3dcentralhessian_": "R:pxx R:pxy R:pxz R:pyy R:pyz R:pzz B:g2 R:phi B:phi B:fb S:dx S:dy S:dz",
+    "lsm3ddeltafunctionorder1_": "R:phi R:delta B:phi R:nx R:ny R:nz B:g B:fb S:dx S:dy S:dz",
     "lsm3dcomputestableadvectiondt_": "SO:dt R:vx R:vy R:vz B:vel B:ib S:dx S:dy S:dz S:cfl",
     "lsm3dcomputestablenormalveldt_": "SO:dt R:vn B:vel R:pxp R:pyp R:pzp B:gp R:pxm R:pym R:pzm B:gm B:ib "
                                       "S:dx S:dy S:dz S:cfl",

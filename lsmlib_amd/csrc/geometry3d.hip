@@ -378,6 +378,35 @@ __global__ void __launch_bounds__(256)
   });
 }
 
+// ---------------------------------------------------------------------------
+// First-order delta function, src/toolbox/lsm_calculus_toolbox3d.f.in:15-141:
+// over the six neighbours n where phi changes sign (phi_n phi <= 0), the sum of
+// |phi_n N / dx / (phi_n - phi)|, N the normal component of the axis -- except
+// that the z terms use norm_phi_x and 1/dx, as the reference's do.
+// ---------------------------------------------------------------------------
+struct DeltaOrder1 {
+  View<real_t> delta;
+  View<const real_t> phi, nx, ny;
+  real_t ix, iy;
+  struct In {
+    real_t c, xp, xm, yp, ym, zp, zm, x, y;
+  };
+  __device__ __forceinline__ In load(int i, int j, int k) const {
+    return In{phi(i, j, k),     phi(i + 1, j, k), phi(i - 1, j, k), phi(i, j + 1, k), phi(i, j - 1, k),
+              phi(i, j, k + 1), phi(i, j, k - 1), nx(i, j, k),      ny(i, j, k)};
+  }
+  __device__ __forceinline__ static real_t side(real_t pn, real_t c, real_t n, real_t inv) {
+    if (!(pn * c <= real_t(0))) return real_t(0);
+    const real_t t = pn * n * inv;
+    return t / (pn - c);
+  }
+  __device__ __forceinline__ void store(int i, int j, int k, const In& v) const {
+    delta(i, j, k) = fabs(side(v.xp, v.c, v.x, ix)) + fabs(side(v.xm, v.c, v.x, ix)) +
+                     fabs(side(v.yp, v.c, v.y, iy)) + fabs(side(v.ym, v.c, v.y, iy)) +
+                     fabs(side(v.zp, v.c, v.x, ix)) + fabs(side(v.zm, v.c, v.x, ix));
+  }
+};
+
 Vec3In vin(const real_t* x, const real_t* y, const real_t* z, const Box& b) {
   return Vec3In{view(x, b), view(y, b), view(z, b)};
 }
@@ -385,7 +414,7 @@ Vec3Out vout(real_t* x, real_t* y, real_t* z, const Box& b) { return Vec3Out{vie
 
 // cells per thread along k
 constexpr int kPointKC = 4;   // normals, mean curvature from a Hessian
-constexpr int kOrthoKC = 2;   // 13 inputs per cell
+constexpr int kOrthoKC = 2;   // 13 inputs per cell (9 for the delta function)
 constexpr int kMarchKC = 16;  // stencil kernels
 
 template <int KC, class Op>
@@ -399,6 +428,20 @@ void launch_pointwise(const Box& fb, const Op& op) {
 // ---------------------------------------------------------------------------
 // Entry points
 // ---------------------------------------------------------------------------
+LSMHIP_EXPORT void lsm3ddeltafunctionorder1_(const real_t* phi, real_t* delta, LSMHIP_BOXP(pg), const real_t* nx,
+                                             const real_t* ny, const real_t* nz, LSMHIP_BOXP(gg), LSMHIP_BOXP(fb),
+                                             const real_t* dx, const real_t* dy, const real_t* dz) {
+  // norm_phi_z is never read (the reference's z terms use norm_phi_x), so it may be NULL
+  LSMHIP_REQUIRE_ARRAYS(phi, delta, nx, ny);
+  (void)nz;
+  const char* who = "lsm3ddeltafunctionorder1_";
+  Box fb = LSMHIP_BOX(fb), pg = LSMHIP_BOX(pg), gg = LSMHIP_BOX(gg);
+  if (fb.empty() || !inside(who, fb, {{gg, "norm_phi"}}) || !cross_inside(who, fb, pg, 1, "phi")) return;
+  launch_pointwise<kOrthoKC>(fb, DeltaOrder1{view(delta, pg), view(phi, pg), view(nx, gg), view(ny, gg),
+                                             real_t(1.0) / *dx, real_t(1.0) / *dy});
+  finish(who);
+}
+
 LSMHIP_EXPORT void lsm3dcomputeunitnormal_(real_t* nx, real_t* ny, real_t* nz, LSMHIP_BOXP(ng), const real_t* px,
                                            const real_t* py, const real_t* pz, LSMHIP_BOXP(gg), LSMHIP_BOXP(fb)) {
   LSMHIP_REQUIRE_ARRAYS(nx, ny, nz, px, py, pz);

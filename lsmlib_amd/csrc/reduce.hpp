@@ -60,7 +60,8 @@ __device__ __forceinline__ Pair wave_sum(Pair v) {
 constexpr int kUnroll = 4;
 
 template <class Op, class Acc, class F>
-__device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[kUnroll], F combine) {
+__device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[kUnroll], F combine,
+                                         double identity = 0.0) {
   const int nx = b.nx(), ny = b.ny();
   const long long rows = (long long)ny * b.nz();
   const int lane = threadIdx.x & 63;
@@ -72,7 +73,7 @@ __device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) {
         const int i = i0 + 64 * u;
-        v[u] = i < nx ? op(b.ilo + i, j, k) : 0.0;
+        v[u] = i < nx ? op(b.ilo + i, j, k) : identity;
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; u++) acc[u] = combine(acc[u], v[u]);
@@ -80,10 +81,16 @@ __device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[
   }
 }
 
-template <class Op>
+// Maximum of op over the box as the IEEE bits of a non-negative double.
+// PRESENT: op returns a negative value for a cell that does not take part;
+// the accumulator then holds 0 when no cell took part and the bits of the
+// maximum plus one otherwise (still in numeric order).
+template <class Op, bool PRESENT = false>
 __global__ void __launch_bounds__(kThreads) max_kernel(Box b, Op op, unsigned long long* acc) {
-  double ms[kUnroll] = {};
-  for_rows(b, op, ms, [](double a, double v) { return v > a ? v : a; });
+  constexpr double kNone = PRESENT ? -1.0 : 0.0;
+  double ms[kUnroll] = {kNone, kNone, kNone, kNone};
+  static_assert(kUnroll == 4, "initialiser above");
+  for_rows(b, op, ms, [](double a, double v) { return v > a ? v : a; }, kNone);
   double m = ms[0];
 #pragma unroll
   for (int u = 1; u < kUnroll; u++) m = ms[u] > m ? ms[u] : m;
@@ -93,7 +100,9 @@ __global__ void __launch_bounds__(kThreads) max_kernel(Box b, Op op, unsigned lo
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kThreads / 64; w++) m = part[w] > m ? part[w] : m;
-    atomicMax(acc, (unsigned long long)__double_as_longlong(m));
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    if (!PRESENT) atomicMax(acc, bits);
+    else if (m >= 0.0) atomicMax(acc, bits + 1ull);
   }
 }
 
@@ -144,16 +153,25 @@ inline int blocks_for(const Box& b) {
 }
 
 // Blocking: the result is needed on the host (the reference returns it through a pointer).
-template <class Op>
+// PRESENT: cells for which op is negative do not take part, and *result is
+// -1 when no cell did.
+template <class Op, bool PRESENT = false>
 bool box_max(const char* who, const Box& b, const Op& op, double* result) {
   unsigned long long* acc = (unsigned long long*)workspace(sizeof(unsigned long long));
   if (!acc) return false;
   if (!check_hip(hipMemsetAsync(acc, 0, sizeof(unsigned long long), stream()), who)) return false;
-  hipLaunchKernelGGL(max_kernel<Op>, dim3(blocks_for(b)), dim3(kThreads), 0, stream(), b, op, acc);
+  hipLaunchKernelGGL((max_kernel<Op, PRESENT>), dim3(blocks_for(b)), dim3(kThreads), 0, stream(), b, op, acc);
   if (!check_hip(hipGetLastError(), who)) return false;
   unsigned long long h = 0;
   if (!check_hip(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, stream()), who)) return false;
   if (!check_hip(hipStreamSynchronize(stream()), who)) return false;
+  if (PRESENT) {
+    if (h == 0) {
+      *result = -1.0;
+      return true;
+    }
+    h -= 1;
+  }
   memcpy(result, &h, sizeof(double));
   return true;
 }

@@ -86,6 +86,71 @@ struct SurfaceTerm {
   }
 };
 
+// The cells of a control volume (src/toolbox/lsm_math_utils3d.f.in:852-1844):
+// control_vol > 0 when control_vol_sgn > 0, control_vol < 0 otherwise.
+struct CvMask {
+  View<const real_t> cv;
+  int sgn;
+  __device__ bool in(int i, int j, int k) const {
+    const real_t c = cv(i, j, k);
+    return sgn > 0 ? c > real_t(0) : c < real_t(0);
+  }
+};
+// A sum term on the control volume's cells only.
+template <class Op>
+struct CvSum {
+  Op op;
+  CvMask m;
+  __device__ double operator()(int i, int j, int k) const { return m.in(i, j, k) ? op(i, j, k) : 0.0; }
+};
+// A maximum term on the control volume's cells only: -1 elsewhere, for the
+// reduction that reports the reference's initial -1 when no cell takes part.
+template <class Op>
+struct CvMax {
+  Op op;
+  CvMask m;
+  __device__ double operator()(int i, int j, int k) const { return m.in(i, j, k) ? op(i, j, k) : -1.0; }
+};
+// The max-norm difference on a control volume starts from the first cell of
+// the box whatever its control volume (lsm_math_utils3d.f.in:887-888), so that
+// cell always takes part.
+struct MaxNormDiffCv {
+  MaxNormDiff d;
+  CvMask m;
+  int ci, cj, ck;
+  __device__ double operator()(int i, int j, int k) const {
+    return m.in(i, j, k) || (i == ci && j == cj && k == ck) ? d(i, j, k) : 0.0;
+  }
+};
+// 1 where phi > 0 (sign > 0) or phi < 0 (sign < 0): the voxel counts; on a
+// control volume these take control_vol >= 0 (sgn > 0) or <= 0
+// (lsm_math_utils3d.f.in:1849-2007).
+struct SignCount {
+  View<const real_t> phi, cv;
+  int sign, sgn;
+  __device__ double operator()(int i, int j, int k) const {
+    const real_t p = phi(i, j, k);
+    if (!(sign > 0 ? p > real_t(0) : p < real_t(0))) return 0.0;
+    if (cv.p) {
+      const real_t c = cv(i, j, k);
+      if (!(sgn > 0 ? c >= real_t(0) : c <= real_t(0))) return 0.0;
+    }
+    return 1.0;
+  }
+};
+// delta_phi |grad phi| dV where a given delta function is positive
+// (lsm_geometry3d_fort.f.in:669-745).
+struct DeltaAreaTerm {
+  View<const real_t> d, x, y, z;
+  real_t dV;
+  __device__ double operator()(int i, int j, int k) const {
+    const real_t dp = d(i, j, k);
+    if (!(dp > real_t(0))) return 0.0;
+    const real_t gx = x(i, j, k), gy = y(i, j, k), gz = z(i, j, k);
+    return (double)(dp * sqrt(gx * gx + gy * gy + gz * gz) * dV);
+  }
+};
+
 bool ok(const char* who, const Box& ib, std::initializer_list<std::pair<Box, const char*>> arrays) {
   for (auto& a : arrays)
     if (!check_box(a.first, ib, who, a.second) || !check_size(a.first, who)) return false;
@@ -124,9 +189,11 @@ LSMHIP_EXPORT void lsm3dcomputestableadvectiondt_(real_t* dt, const real_t* vx, 
   *dt = cfl_dt(m, *cfl);
 }
 
+// cv: the control volume whose cells take part (all of ib when null).
 static void normal_dt(const char* who, real_t* dt, const real_t* vn, const Box& vg, real_t vconst, const real_t* pxp,
                       const real_t* pyp, const real_t* pzp, const Box& gp, const real_t* pxm, const real_t* pym,
-                      const real_t* pzm, const Box& gm, const Box& ib, real_t dx, real_t dy, real_t dz, real_t cfl) {
+                      const real_t* pzm, const Box& gm, const Box& ib, real_t dx, real_t dy, real_t dz, real_t cfl,
+                      const CvMask* cv = nullptr) {
   double m = -1.0;
   if (!ib.empty()) {
     if (!ok(who, ib, {{gp, "grad_phi_plus"}, {gm, "grad_phi_minus"}})) return;
@@ -136,7 +203,9 @@ static void normal_dt(const char* who, real_t* dt, const real_t* vn, const Box& 
     NormalCfl op{vn ? view(vn, vg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0},
                  view(pxp, gp), view(pyp, gp), view(pzp, gp), view(pxm, gm), view(pym, gm), view(pzm, gm),
                  vconst, h * h, real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz};
-    if (!reduce::box_max(who, ib, op, &m)) return;
+    if (cv ? !reduce::box_max<CvMax<NormalCfl>, true>(who, ib, CvMax<NormalCfl>{op, *cv}, &m)
+           : !reduce::box_max(who, ib, op, &m))
+      return;
   }
   *dt = cfl_dt(m, cfl);
 }
@@ -165,9 +234,23 @@ LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldt_(real_t* dt, const real_t*
 // ----- integrals -----------------------------------------------------------
 // buf: device memory for the sum's partials (2 * reduce::kMaxBlocks + 2
 // doubles), or null for the library workspace
+// A sum over ib of op, or of op on the control volume's cells when cv is set.
+template <class Op>
+static bool sum_over(const char* who, const Box& ib, const Op& op, const CvMask* cv, double* s, double* buf = nullptr) {
+  if (cv) return reduce::box_sum(who, ib, CvSum<Op>{op, *cv}, s, buf);
+  return reduce::box_sum(who, ib, op, s, buf);
+}
+
+// A control volume's mask after its array has been checked against ib.
+static bool cv_mask(const char* who, const Box& ib, const real_t* cv, const Box& cg, int sgn, CvMask* m) {
+  if (!ok(who, ib, {{cg, "control_vol"}})) return false;
+  *m = CvMask{view(cv, cg), sgn};
+  return true;
+}
+
 static double volume(const char* who, const real_t* F, const Box& fg, const real_t* phi, const Box& pg,
                      const Box& ib, real_t dx, real_t dy, real_t dz, real_t eps, int sign, bool* good,
-                     double* buf = nullptr) {
+                     double* buf = nullptr, const CvMask* cv = nullptr) {
   *good = false;
   if (ib.empty()) {
     *good = true;
@@ -178,13 +261,13 @@ static double volume(const char* who, const real_t* F, const Box& fg, const real
   VolumeTerm op{view(phi, pg), F ? view(F, fg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, eps, dx * dy * dz,
                 sign};
   double s = 0;
-  *good = reduce::box_sum(who, ib, op, &s, buf);
+  *good = sum_over(who, ib, op, cv, &s, buf);
   return s;
 }
 
 static double surface(const char* who, const real_t* F, const Box& fg, const real_t* phi, const Box& pg,
                       const real_t* px, const real_t* py, const real_t* pz, const Box& g, const Box& ib, real_t dx,
-                      real_t dy, real_t dz, real_t eps, bool* good) {
+                      real_t dy, real_t dz, real_t eps, bool* good, const CvMask* cv = nullptr) {
   *good = false;
   if (ib.empty()) {
     *good = true;
@@ -195,7 +278,7 @@ static double surface(const char* who, const real_t* F, const Box& fg, const rea
   SurfaceTerm op{view(phi, pg), F ? view(F, fg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, view(px, g),
                  view(py, g), view(pz, g), eps, real_t(1.0) / eps, dx * dy * dz};
   double s = 0;
-  *good = reduce::box_sum(who, ib, op, &s);
+  *good = sum_over(who, ib, op, cv, &s);
   return s;
 }
 
@@ -260,6 +343,231 @@ LSMHIP_EXPORT void lsm3dsurfaceareazerolevelset_(real_t* out, const real_t* phi,
                      LSMHIP_BOX(ib), *dx, *dy, *dz, *eps, &good);
   if (good) *out = real_t(s);
 }
+
+// ---------------------------------------------------------------------------
+// Average difference, voxel counts, area from a delta function
+// ---------------------------------------------------------------------------
+LSMHIP_EXPORT void lsm3daveabsdiff_(real_t* out, const real_t* f1, LSMHIP_BOXP(g1), const real_t* f2, LSMHIP_BOXP(g2),
+                                    LSMHIP_BOXP(ib)) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2);
+  const char* who = "lsm3daveabsdiff_";
+  const Box ib = LSMHIP_BOX(ib), g1 = LSMHIP_BOX(g1), g2 = LSMHIP_BOX(g2);
+  if (ib.empty()) {
+    *out = real_t(0);
+    return;
+  }
+  if (!ok(who, ib, {{g1, "field1"}, {g2, "field2"}})) return;
+  double s = 0;
+  if (!reduce::box_sum(who, ib, MaxNormDiff{view(f1, g1), view(f2, g2)}, &s)) return;
+  // the sum rounded to the storage precision over the (exact) number of cells
+  *out = real_t(s) / real_t((double)ib.nx() * ib.ny() * ib.nz());
+}
+
+static void voxel_count(const char* who, int* count, const real_t* phi, const Box& pg, const real_t* cv,
+                        const Box& cg, int sgn, const Box& fb, int sign) {
+  if (fb.empty()) {
+    *count = 0;
+    return;
+  }
+  if (!ok(who, fb, {{pg, "phi"}})) return;
+  if (cv && !ok(who, fb, {{cg, "control_vol"}})) return;
+  SignCount op{view(phi, pg), cv ? view(cv, cg) : View<const real_t>{nullptr, 0, 0, 0, 0, 0}, sign, sgn};
+  double s = 0;
+  if (reduce::box_sum(who, fb, op, &s)) *count = (int)s;
+}
+LSMHIP_EXPORT void lsm3dvoxelcountgreaterthanzero_(int* count, const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(count, phi);
+  voxel_count("lsm3dvoxelcountgreaterthanzero_", count, phi, LSMHIP_BOX(pg), nullptr, Box{0, -1, 0, -1, 0, -1}, 0,
+              LSMHIP_BOX(fb), +1);
+}
+LSMHIP_EXPORT void lsm3dvoxelcountlessthanzero_(int* count, const real_t* phi, LSMHIP_BOXP(pg), LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(count, phi);
+  voxel_count("lsm3dvoxelcountlessthanzero_", count, phi, LSMHIP_BOX(pg), nullptr, Box{0, -1, 0, -1, 0, -1}, 0,
+              LSMHIP_BOX(fb), -1);
+}
+
+static void delta_area(const char* who, real_t* out, const real_t* dphi, const Box& pg, const real_t* px,
+                       const real_t* py, const real_t* pz, const Box& g, const Box& ib, real_t dx, real_t dy,
+                       real_t dz, const CvMask* cv) {
+  double s = 0;
+  if (!ib.empty()) {
+    if (!ok(who, ib, {{pg, "delta_phi"}, {g, "grad_phi"}})) return;
+    DeltaAreaTerm op{view(dphi, pg), view(px, g), view(py, g), view(pz, g), dx * dy * dz};
+    if (!sum_over(who, ib, op, cv, &s)) return;
+  }
+  *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceareazerolevelsetdelta_(real_t* out, const real_t* dphi, LSMHIP_BOXP(pg),
+                                                      const real_t* px, const real_t* py, const real_t* pz,
+                                                      LSMHIP_BOXP(g), LSMHIP_BOXP(ib), const real_t* dx,
+                                                      const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(dphi, px, py, pz);
+  delta_area("lsm3dsurfaceareazerolevelsetdelta_", out, dphi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g),
+             LSMHIP_BOX(ib), *dx, *dy, *dz, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Control-volume forms (src/toolbox/lsm_math_utils3d.f.in:852-2007,
+// src/geometry/lsm_geometry3d_fort.f.in:770-1329): the whole-box computation
+// on the cells of ib inside the control volume.
+// ---------------------------------------------------------------------------
+#define CV_ARGS const real_t *cv, LSMHIP_BOXP(cg), const int *cvs
+#define CV_MASK(who)                                                                    \
+  CvMask m{};                                                                           \
+  const Box ib_ = LSMHIP_BOX(ib);                                                       \
+  if (!ib_.empty() && !cv_mask(who, ib_, cv, LSMHIP_BOX(cg), *cvs, &m)) return
+
+LSMHIP_EXPORT void lsm3dmaxnormdiffcontrolvolume_(real_t* out, const real_t* f1, LSMHIP_BOXP(g1), const real_t* f2,
+                                                  LSMHIP_BOXP(g2), CV_ARGS, LSMHIP_BOXP(ib)) {
+  LSMHIP_REQUIRE_ARRAYS(f1, f2, cv);
+  const char* who = "lsm3dmaxnormdiffcontrolvolume_";
+  const Box ib = LSMHIP_BOX(ib), g1 = LSMHIP_BOX(g1), g2 = LSMHIP_BOX(g2);
+  *out = real_t(0);
+  if (ib.empty() || !ok(who, ib, {{g1, "field1"}, {g2, "field2"}})) return;
+  CvMask m;
+  if (!cv_mask(who, ib, cv, LSMHIP_BOX(cg), *cvs, &m)) return;
+  double r = 0;
+  MaxNormDiffCv op{MaxNormDiff{view(f1, g1), view(f2, g2)}, m, ib.ilo, ib.jlo, ib.klo};
+  if (reduce::box_max(who, ib, op, &r)) *out = real_t(r);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestableadvectiondtcontrolvolume_(real_t* dt, const real_t* vx, const real_t* vy,
+                                                               const real_t* vz, LSMHIP_BOXP(vg), CV_ARGS,
+                                                               LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                                               const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vx, vy, vz, cv);
+  const char* who = "lsm3dcomputestableadvectiondtcontrolvolume_";
+  const Box ib = LSMHIP_BOX(ib), vg = LSMHIP_BOX(vg);
+  double m = -1.0;
+  if (!ib.empty()) {
+    if (!ok(who, ib, {{vg, "vel"}})) return;
+    CvMask cm;
+    if (!cv_mask(who, ib, cv, LSMHIP_BOX(cg), *cvs, &cm)) return;
+    AdvectionCfl op{view(vx, vg), view(vy, vg), view(vz, vg), real_t(1.0) / *dx, real_t(1.0) / *dy,
+                    real_t(1.0) / *dz};
+    if (!reduce::box_max<CvMax<AdvectionCfl>, true>(who, ib, CvMax<AdvectionCfl>{op, cm}, &m)) return;
+  }
+  *dt = cfl_dt(m, *cfl);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestablenormalveldtcontrolvolume_(
+    real_t* dt, const real_t* vn, LSMHIP_BOXP(vg), const real_t* pxp, const real_t* pyp, const real_t* pzp,
+    LSMHIP_BOXP(gp), const real_t* pxm, const real_t* pym, const real_t* pzm, LSMHIP_BOXP(gm), CV_ARGS,
+    LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy, const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(vn, pxp, pyp, pzp, pxm, pym, pzm, cv);
+  const char* who = "lsm3dcomputestablenormalveldtcontrolvolume_";
+  CV_MASK(who);
+  normal_dt(who, dt, vn, LSMHIP_BOX(vg), real_t(0), pxp, pyp, pzp, LSMHIP_BOX(gp), pxm, pym, pzm, LSMHIP_BOX(gm),
+            ib_, *dx, *dy, *dz, *cfl, &m);
+}
+
+LSMHIP_EXPORT void lsm3dcomputestableconstnormalveldtcontrolvolume_(
+    real_t* dt, const real_t* vel_n, const real_t* pxp, const real_t* pyp, const real_t* pzp, LSMHIP_BOXP(gp),
+    const real_t* pxm, const real_t* pym, const real_t* pzm, LSMHIP_BOXP(gm), CV_ARGS, LSMHIP_BOXP(ib),
+    const real_t* dx, const real_t* dy, const real_t* dz, const real_t* cfl) {
+  LSMHIP_REQUIRE_ARRAYS(pxp, pyp, pzp, pxm, pym, pzm, cv);
+  const char* who = "lsm3dcomputestableconstnormalveldtcontrolvolume_";
+  CV_MASK(who);
+  normal_dt(who, dt, nullptr, Box{0, -1, 0, -1, 0, -1}, real_t(fabs(*vel_n)), pxp, pyp, pzp, LSMHIP_BOX(gp), pxm,
+            pym, pzm, LSMHIP_BOX(gm), ib_, *dx, *dy, *dz, *cfl, &m);
+}
+
+LSMHIP_EXPORT void lsm3dvolumeintegralphilessthanzerocontrolvolume_(real_t* out, const real_t* F, LSMHIP_BOXP(fg),
+                                                                    const real_t* phi, LSMHIP_BOXP(pg), CV_ARGS,
+                                                                    LSMHIP_BOXP(ib), const real_t* dx,
+                                                                    const real_t* dy, const real_t* dz,
+                                                                    const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi, cv);
+  const char* who = "lsm3dvolumeintegralphilessthanzerocontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = volume(who, F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), ib_, *dx, *dy, *dz, *eps, -1, &good, nullptr, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvolumeintegralphigreaterthanzerocontrolvolume_(
+    real_t* out, const real_t* F, LSMHIP_BOXP(fg), const real_t* phi, LSMHIP_BOXP(pg), CV_ARGS, LSMHIP_BOXP(ib),
+    const real_t* dx, const real_t* dy, const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi, cv);
+  const char* who = "lsm3dvolumeintegralphigreaterthanzerocontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = volume(who, F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), ib_, *dx, *dy, *dz, *eps, +1, &good, nullptr, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceintegralcontrolvolume_(real_t* out, const real_t* F, LSMHIP_BOXP(fg),
+                                                      const real_t* phi, LSMHIP_BOXP(pg), const real_t* px,
+                                                      const real_t* py, const real_t* pz, LSMHIP_BOXP(g), CV_ARGS,
+                                                      LSMHIP_BOXP(ib), const real_t* dx, const real_t* dy,
+                                                      const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(F, phi, px, py, pz, cv);
+  const char* who = "lsm3dsurfaceintegralcontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = surface(who, F, LSMHIP_BOX(fg), phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g), ib_, *dx, *dy, *dz,
+                     *eps, &good, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvoxelcountgreaterthanzerocontrolvolume_(int* count, const real_t* phi, LSMHIP_BOXP(pg),
+                                                                CV_ARGS, LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(count, phi, cv);
+  voxel_count("lsm3dvoxelcountgreaterthanzerocontrolvolume_", count, phi, LSMHIP_BOX(pg), cv, LSMHIP_BOX(cg), *cvs,
+              LSMHIP_BOX(fb), +1);
+}
+LSMHIP_EXPORT void lsm3dvoxelcountlessthanzerocontrolvolume_(int* count, const real_t* phi, LSMHIP_BOXP(pg),
+                                                             CV_ARGS, LSMHIP_BOXP(fb)) {
+  LSMHIP_REQUIRE_ARRAYS(count, phi, cv);
+  voxel_count("lsm3dvoxelcountlessthanzerocontrolvolume_", count, phi, LSMHIP_BOX(pg), cv, LSMHIP_BOX(cg), *cvs,
+              LSMHIP_BOX(fb), -1);
+}
+LSMHIP_EXPORT void lsm3dvolumeregionphilessthanzerocontrolvolume_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
+                                                                  CV_ARGS, LSMHIP_BOXP(ib), const real_t* dx,
+                                                                  const real_t* dy, const real_t* dz,
+                                                                  const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi, cv);
+  const char* who = "lsm3dvolumeregionphilessthanzerocontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = volume(who, nullptr, Box{0, -1, 0, -1, 0, -1}, phi, LSMHIP_BOX(pg), ib_, *dx, *dy, *dz, *eps, -1, &good,
+                    nullptr, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dvolumeregionphigreaterthanzerocontrolvolume_(real_t* out, const real_t* phi,
+                                                                     LSMHIP_BOXP(pg), CV_ARGS, LSMHIP_BOXP(ib),
+                                                                     const real_t* dx, const real_t* dy,
+                                                                     const real_t* dz, const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi, cv);
+  const char* who = "lsm3dvolumeregionphigreaterthanzerocontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = volume(who, nullptr, Box{0, -1, 0, -1, 0, -1}, phi, LSMHIP_BOX(pg), ib_, *dx, *dy, *dz, *eps, +1, &good,
+                    nullptr, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceareazerolevelsetcontrolvolume_(real_t* out, const real_t* phi, LSMHIP_BOXP(pg),
+                                                              const real_t* px, const real_t* py, const real_t* pz,
+                                                              LSMHIP_BOXP(g), CV_ARGS, LSMHIP_BOXP(ib),
+                                                              const real_t* dx, const real_t* dy, const real_t* dz,
+                                                              const real_t* eps) {
+  LSMHIP_REQUIRE_ARRAYS(phi, px, py, pz, cv);
+  const char* who = "lsm3dsurfaceareazerolevelsetcontrolvolume_";
+  CV_MASK(who);
+  bool good;
+  double s = surface(who, nullptr, Box{0, -1, 0, -1, 0, -1}, phi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g), ib_,
+                     *dx, *dy, *dz, *eps, &good, &m);
+  if (good) *out = real_t(s);
+}
+LSMHIP_EXPORT void lsm3dsurfaceareazerolevelsetdeltacontrolvolume_(real_t* out, const real_t* dphi, LSMHIP_BOXP(pg),
+                                                                   const real_t* px, const real_t* py,
+                                                                   const real_t* pz, LSMHIP_BOXP(g), CV_ARGS,
+                                                                   LSMHIP_BOXP(ib), const real_t* dx,
+                                                                   const real_t* dy, const real_t* dz) {
+  LSMHIP_REQUIRE_ARRAYS(dphi, px, py, pz, cv);
+  const char* who = "lsm3dsurfaceareazerolevelsetdeltacontrolvolume_";
+  CV_MASK(who);
+  delta_area(who, out, dphi, LSMHIP_BOX(pg), px, py, pz, LSMHIP_BOX(g), ib_, *dx, *dy, *dz, &m);
+}
+#undef CV_MASK
+#undef CV_ARGS
 
 // FP64-accumulated volume of {phi < 0} returned in double whatever the storage
 // precision (used by the FP32 path for its interface-volume error).

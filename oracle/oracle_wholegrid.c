@@ -1007,3 +1007,358 @@ void oracle_lsmdeltafunction(double *delta, const double *x, const double *epsil
     *delta = 0.5 * ie * (1.0 + cos(ORC_PI * v * ie));
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * Averages, voxel counts and control-volume forms, src/toolbox/lsm_math_utils3d.f.in
+ * and src/geometry/lsm_geometry3d_fort.f.in.  A control-volume routine visits
+ * the cells of ib whose control_vol is > 0 (control_vol_sgn > 0) or < 0
+ * (otherwise) and does there what its whole-box twin does.
+ * ------------------------------------------------------------------------- */
+#define CV_ARGS const double *cv, BOXP(cg), const int *cvs
+#define CV_VIEW orc_arr CV = VIEW(cv, cg)
+#define CV_IN (*cvs > 0 ? AT(CV, i, j, k) > 0.0 : AT(CV, i, j, k) < 0.0)
+
+/* lsm_math_utils3d.f.in:81-148: serial sum of |f1 - f2| and count, then their
+ * quotient (0 for an empty box) */
+void oracle_lsm3daveabsdiff(double *out, const double *f1, BOXP(g1), const double *f2, BOXP(g2), BOXP(ib)) {
+  orc_arr A = VIEW(f1, g1), B = VIEW(f2, g2);
+  double s = 0.0, n = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    s = s + fabs(AT(A, i, j, k) - AT(B, i, j, k));
+    n = n + 1.0;
+  }
+  *out = n > 0.0 ? s / n : 0.0;
+}
+
+/* :154-198 and :203-247 */
+void oracle_lsm3dvoxelcountgreaterthanzero(int *count, const double *phi, BOXP(pg), BOXP(fb)) {
+  orc_arr P = VIEW(phi, pg);
+  int c = 0;
+  obox b = BOX(fb);
+  FOR_FB(b) if (AT(P, i, j, k) > 0.0) c++;
+  *count = c;
+}
+void oracle_lsm3dvoxelcountlessthanzero(int *count, const double *phi, BOXP(pg), BOXP(fb)) {
+  orc_arr P = VIEW(phi, pg);
+  int c = 0;
+  obox b = BOX(fb);
+  FOR_FB(b) if (AT(P, i, j, k) < 0.0) c++;
+  *count = c;
+}
+
+/* :852-955: starts from the first include-box cell whatever its control volume */
+void oracle_lsm3dmaxnormdiffcontrolvolume(double *out, const double *f1, BOXP(g1), const double *f2, BOXP(g2),
+                                          CV_ARGS, BOXP(ib)) {
+  orc_arr A = VIEW(f1, g1), B = VIEW(f2, g2);
+  CV_VIEW;
+  double m = fabs(AT(A, *ilo_ib, *jlo_ib, *klo_ib) - AT(B, *ilo_ib, *jlo_ib, *klo_ib));
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double d = fabs(AT(A, i, j, k) - AT(B, i, j, k));
+    if (d > m) m = d;
+  }
+  *out = m;
+}
+
+/* :960-1075 */
+void oracle_lsm3dcomputestableadvectiondtcontrolvolume(double *dt, const double *vx, const double *vy,
+                                                       const double *vz, BOXP(vg), CV_ARGS, BOXP(ib),
+                                                       const double *dx, const double *dy, const double *dz,
+                                                       const double *cfl) {
+  orc_arr U = VIEW(vx, vg), V = VIEW(vy, vg), W = VIEW(vz, vg);
+  CV_VIEW;
+  double m = -1.0, ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double c = fabs(AT(U, i, j, k)) * ix + fabs(AT(V, i, j, k)) * iy + fabs(AT(W, i, j, k)) * iz;
+    if (c > m) m = c;
+  }
+  *dt = *cfl / (m + ORC_TINY);
+}
+
+/* :1080-1250 (vel_n a field) and :1255-1415 (a constant) */
+static void normal_dt_cv(double *dt, const double *vn, orc_arr VN, double vconst, const double *pxp,
+                         const double *pyp, const double *pzp, BOXP(gp), const double *pxm, const double *pym,
+                         const double *pzm, BOXP(gm), CV_ARGS, BOXP(ib), const double *dx, const double *dy,
+                         const double *dz, const double *cfl) {
+  orc_arr XP = VIEW(pxp, gp), YP = VIEW(pyp, gp), ZP = VIEW(pzp, gp);
+  orc_arr XM = VIEW(pxm, gm), YM = VIEW(pym, gm), ZM = VIEW(pzm, gm);
+  CV_VIEW;
+  double h = orc_max(orc_max(*dx, *dy), *dz), mdx2 = h * h;
+  double m = -1.0, ix = 1.0 / *dx, iy = 1.0 / *dy, iz = 1.0 / *dz;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double av = vn ? fabs(AT(VN, i, j, k)) : vconst;
+    double c = normal_h_over_dx(av, AT(XP, i, j, k), AT(YP, i, j, k), AT(ZP, i, j, k), AT(XM, i, j, k),
+                                AT(YM, i, j, k), AT(ZM, i, j, k), mdx2, ix, iy, iz);
+    if (c > m) m = c;
+  }
+  *dt = *cfl / (m + ORC_TINY);
+}
+void oracle_lsm3dcomputestablenormalveldtcontrolvolume(double *dt, const double *vn, BOXP(vg), const double *pxp,
+                                                       const double *pyp, const double *pzp, BOXP(gp),
+                                                       const double *pxm, const double *pym, const double *pzm,
+                                                       BOXP(gm), CV_ARGS, BOXP(ib), const double *dx,
+                                                       const double *dy, const double *dz, const double *cfl) {
+  normal_dt_cv(dt, vn, VIEW(vn, vg), 0.0, pxp, pyp, pzp, ilo_gp, ihi_gp, jlo_gp, jhi_gp, klo_gp, khi_gp, pxm, pym,
+               pzm, ilo_gm, ihi_gm, jlo_gm, jhi_gm, klo_gm, khi_gm, cv, ilo_cg, ihi_cg, jlo_cg, jhi_cg, klo_cg,
+               khi_cg, cvs, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib, dx, dy, dz, cfl);
+}
+void oracle_lsm3dcomputestableconstnormalveldtcontrolvolume(double *dt, const double *vel_n, const double *pxp,
+                                                            const double *pyp, const double *pzp, BOXP(gp),
+                                                            const double *pxm, const double *pym,
+                                                            const double *pzm, BOXP(gm), CV_ARGS, BOXP(ib),
+                                                            const double *dx, const double *dy, const double *dz,
+                                                            const double *cfl) {
+  orc_arr none = {0};
+  normal_dt_cv(dt, (const double *)0, none, fabs(*vel_n), pxp, pyp, pzp, ilo_gp, ihi_gp, jlo_gp, jhi_gp, klo_gp, khi_gp, pxm,
+               pym, pzm, ilo_gm, ihi_gm, jlo_gm, jhi_gm, klo_gm, khi_gm, cv, ilo_cg, ihi_cg, jlo_cg, jhi_cg, klo_cg,
+               khi_cg, cvs, ilo_ib, ihi_ib, jlo_ib, jhi_ib, klo_ib, khi_ib, dx, dy, dz, cfl);
+}
+
+/* :1420-1551 and :1556-1686 (naive serial sums, as the reference's) */
+void oracle_lsm3dvolumeintegralphilessthanzerocontrolvolume(double *intF, const double *F, BOXP(fg),
+                                                            const double *phi, BOXP(pg), CV_ARGS, BOXP(ib),
+                                                            const double *dx, const double *dy, const double *dz,
+                                                            const double *epsilon) {
+  orc_arr FF = VIEW(F, fg), P = VIEW(phi, pg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k), q = p / e;
+    if (p < -e) s = s + AT(FF, i, j, k) * dV;
+    else if (p < e) s = s + 0.5 * (1.0 - q - ORC_ONE_OVER_PI * sin(ORC_PI * q)) * AT(FF, i, j, k) * dV;
+  }
+  *intF = s;
+}
+void oracle_lsm3dvolumeintegralphigreaterthanzerocontrolvolume(double *intF, const double *F, BOXP(fg),
+                                                               const double *phi, BOXP(pg), CV_ARGS, BOXP(ib),
+                                                               const double *dx, const double *dy,
+                                                               const double *dz, const double *epsilon) {
+  orc_arr FF = VIEW(F, fg), P = VIEW(phi, pg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k), q = p / e;
+    if (p > e) s = s + AT(FF, i, j, k) * dV;
+    else if (p > -e) s = s + 0.5 * (1.0 + q + ORC_ONE_OVER_PI * sin(ORC_PI * q)) * AT(FF, i, j, k) * dV;
+  }
+  *intF = s;
+}
+
+/* :1691-1844 */
+void oracle_lsm3dsurfaceintegralcontrolvolume(double *intF, const double *F, BOXP(fg), const double *phi,
+                                              BOXP(pg), const double *px, const double *py, const double *pz,
+                                              BOXP(gg), CV_ARGS, BOXP(ib), const double *dx, const double *dy,
+                                              const double *dz, const double *epsilon) {
+  orc_arr FF = VIEW(F, fg), P = VIEW(phi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, ie = 1.0 / e, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k);
+    if (fabs(p) < e) {
+      double delta = 0.5 * ie * (1.0 + cos(ORC_PI * p * ie));
+      double x = AT(X, i, j, k), y = AT(Y, i, j, k), z = AT(Z, i, j, k);
+      double n = sqrt(x * x + y * y + z * z);
+      s = s + delta * n * AT(FF, i, j, k) * dV;
+    }
+  }
+  *intF = s;
+}
+
+/* :1849-1925 and :1931-2007: these two take control_vol >= 0 (sgn > 0) and
+ * <= 0 (otherwise) */
+void oracle_lsm3dvoxelcountgreaterthanzerocontrolvolume(int *count, const double *phi, BOXP(pg), CV_ARGS,
+                                                        BOXP(fb)) {
+  orc_arr P = VIEW(phi, pg);
+  CV_VIEW;
+  int c = 0;
+  obox b = BOX(fb);
+  FOR_FB(b) {
+    double v = AT(CV, i, j, k);
+    if (AT(P, i, j, k) > 0.0 && (*cvs > 0 ? v >= 0.0 : v <= 0.0)) c++;
+  }
+  *count = c;
+}
+void oracle_lsm3dvoxelcountlessthanzerocontrolvolume(int *count, const double *phi, BOXP(pg), CV_ARGS,
+                                                     BOXP(fb)) {
+  orc_arr P = VIEW(phi, pg);
+  CV_VIEW;
+  int c = 0;
+  obox b = BOX(fb);
+  FOR_FB(b) {
+    double v = AT(CV, i, j, k);
+    if (AT(P, i, j, k) < 0.0 && (*cvs > 0 ? v >= 0.0 : v <= 0.0)) c++;
+  }
+  *count = c;
+}
+
+/* lsm_geometry3d_fort.f.in:669-745: delta_phi |grad phi| dV over the cells
+ * where delta_phi > 0 */
+void oracle_lsm3dsurfaceareazerolevelsetdelta(double *area, const double *dphi, BOXP(pg), const double *px,
+                                              const double *py, const double *pz, BOXP(gg), BOXP(ib),
+                                              const double *dx, const double *dy, const double *dz) {
+  orc_arr D = VIEW(dphi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg);
+  double dV = *dx * *dy * *dz, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    double d = AT(D, i, j, k);
+    if (d > 0.0) {
+      double x = AT(X, i, j, k), y = AT(Y, i, j, k), z = AT(Z, i, j, k);
+      s = s + d * sqrt(x * x + y * y + z * z) * dV;
+    }
+  }
+  *area = s;
+}
+
+/* :770-892 and :916-1035 */
+void oracle_lsm3dvolumeregionphilessthanzerocontrolvolume(double *vol, const double *phi, BOXP(pg), CV_ARGS,
+                                                          BOXP(ib), const double *dx, const double *dy,
+                                                          const double *dz, const double *epsilon) {
+  orc_arr P = VIEW(phi, pg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k);
+    if (p < e) s = s + vol_lt0_term(p, e, dV);
+  }
+  *vol = s;
+}
+void oracle_lsm3dvolumeregionphigreaterthanzerocontrolvolume(double *vol, const double *phi, BOXP(pg), CV_ARGS,
+                                                             BOXP(ib), const double *dx, const double *dy,
+                                                             const double *dz, const double *epsilon) {
+  orc_arr P = VIEW(phi, pg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k);
+    if (p > -e) s = s + vol_gt0_term(p, e, dV);
+  }
+  *vol = s;
+}
+
+/* :1061-1204 */
+void oracle_lsm3dsurfaceareazerolevelsetcontrolvolume(double *area, const double *phi, BOXP(pg),
+                                                      const double *px, const double *py, const double *pz,
+                                                      BOXP(gg), CV_ARGS, BOXP(ib), const double *dx,
+                                                      const double *dy, const double *dz, const double *epsilon) {
+  orc_arr P = VIEW(phi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, e = *epsilon, ie = 1.0 / e, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    if (!CV_IN) continue;
+    double p = AT(P, i, j, k);
+    if (fabs(p) < e) s = s + area_term(p, AT(X, i, j, k), AT(Y, i, j, k), AT(Z, i, j, k), e, ie, dV);
+  }
+  *area = s;
+}
+
+/* :1209-1329 */
+void oracle_lsm3dsurfaceareazerolevelsetdeltacontrolvolume(double *area, const double *dphi, BOXP(pg),
+                                                           const double *px, const double *py, const double *pz,
+                                                           BOXP(gg), CV_ARGS, BOXP(ib), const double *dx,
+                                                           const double *dy, const double *dz) {
+  orc_arr D = VIEW(dphi, pg), X = VIEW(px, gg), Y = VIEW(py, gg), Z = VIEW(pz, gg);
+  CV_VIEW;
+  double dV = *dx * *dy * *dz, s = 0.0;
+  obox b = BOX(ib);
+  FOR_FB(b) {
+    double d = AT(D, i, j, k);
+    if (CV_IN && d > 0.0) {
+      double x = AT(X, i, j, k), y = AT(Y, i, j, k), z = AT(Z, i, j, k);
+      s = s + d * sqrt(x * x + y * y + z * z) * dV;
+    }
+  }
+  *area = s;
+}
+
+/* ---------------------------------------------------------------------------
+ * Hessian, phi-upwind gradient, first-order delta function
+ * ------------------------------------------------------------------------- */
+/* lsm3dCentralHessian, src/toolbox/lsm_spatial_derivatives3d.f.in:3658-3715 */
+void oracle_lsm3dcentralhessian(double *pxx, double *pxy, double *pxz, double *pyy, double *pyz, double *pzz,
+                                BOXP(hg), const double *phi, BOXP(pg), BOXP(fb), const double *dx,
+                                const double *dy, const double *dz) {
+  orc_arr XX = VIEW(pxx, hg), XY = VIEW(pxy, hg), XZ = VIEW(pxz, hg), YY = VIEW(pyy, hg), YZ = VIEW(pyz, hg),
+          ZZ = VIEW(pzz, hg), P = VIEW(phi, pg);
+  obox b = BOX(fb);
+  FOR_FB(b) {
+    double c = AT(P, i, j, k);
+    AT(XX, i, j, k) = (AT(P, i + 1, j, k) - 2.0 * c + AT(P, i - 1, j, k)) / (*dx * *dx);
+    AT(XY, i, j, k) = 0.25 * (AT(P, i + 1, j + 1, k) - AT(P, i - 1, j + 1, k) - AT(P, i + 1, j - 1, k) +
+                              AT(P, i - 1, j - 1, k)) / (*dx * *dy);
+    AT(XZ, i, j, k) = 0.25 * (AT(P, i + 1, j, k + 1) - AT(P, i - 1, j, k + 1) - AT(P, i + 1, j, k - 1) +
+                              AT(P, i - 1, j, k - 1)) / (*dx * *dz);
+    AT(YY, i, j, k) = (AT(P, i, j + 1, k) - 2.0 * c + AT(P, i, j - 1, k)) / (*dy * *dy);
+    AT(YZ, i, j, k) = 0.25 * (AT(P, i, j + 1, k + 1) - AT(P, i, j - 1, k + 1) - AT(P, i, j + 1, k - 1) +
+                              AT(P, i, j - 1, k - 1)) / (*dy * *dz);
+    AT(ZZ, i, j, k) = (AT(P, i, j, k + 1) - 2.0 * c + AT(P, i, j, k - 1)) / (*dz * *dz);
+  }
+}
+
+/* One axis of lsm3dPhiUpwindGradF (:3270-3458): the side whose phi is nearer
+ * zero (smaller for phi > 0, larger for phi < 0), the plus side on a tie of
+ * the two neighbours only where the minus side does not qualify. */
+static double phi_upwind_pick(double c, double nm, double np, double fp, double fm) {
+  if (c > 0.0) {
+    if (nm <= c) return np < nm ? fp : fm;
+    if (np <= c) return fp;
+    return 0.0;
+  }
+  if (nm >= c) return np > nm ? fp : fm;
+  if (np >= c) return fp;
+  return 0.0;
+}
+void oracle_lsm3dphiupwindgradf(double *Fx, double *Fy, double *Fz, BOXP(g), const double *Fxp,
+                                const double *Fyp, const double *Fzp, BOXP(gp), const double *Fxm,
+                                const double *Fym, const double *Fzm, BOXP(gm), const double *phi, BOXP(pg),
+                                BOXP(fb)) {
+  orc_arr X = VIEW(Fx, g), Y = VIEW(Fy, g), Z = VIEW(Fz, g), XP = VIEW(Fxp, gp), YP = VIEW(Fyp, gp),
+          ZP = VIEW(Fzp, gp), XM = VIEW(Fxm, gm), YM = VIEW(Fym, gm), ZM = VIEW(Fzm, gm), P = VIEW(phi, pg);
+  obox b = BOX(fb);
+  FOR_FB(b) {
+    double c = AT(P, i, j, k);
+    if (!(c > 0.0) && !(c < 0.0)) continue; /* phi == 0 (or NaN): left as is */
+    AT(X, i, j, k) = phi_upwind_pick(c, AT(P, i - 1, j, k), AT(P, i + 1, j, k), AT(XP, i, j, k), AT(XM, i, j, k));
+    AT(Y, i, j, k) = phi_upwind_pick(c, AT(P, i, j - 1, k), AT(P, i, j + 1, k), AT(YP, i, j, k), AT(YM, i, j, k));
+    AT(Z, i, j, k) = phi_upwind_pick(c, AT(P, i, j, k - 1), AT(P, i, j, k + 1), AT(ZP, i, j, k), AT(ZM, i, j, k));
+  }
+}
+
+/* lsm3dDeltaFunctionOrder1, src/toolbox/lsm_calculus_toolbox3d.f.in:15-141: the
+ * sum of |phi_n N / dx / (phi_n - phi)| over the six neighbours n where phi
+ * changes sign (phi_n phi <= 0); the z terms use norm_phi_x and 1/dx. */
+static double delta_side(double pn, double c, double nrm, double inv) {
+  if (!(pn * c <= 0.0)) return 0.0;
+  double t = pn * nrm * inv;
+  return t / (pn - c);
+}
+void oracle_lsm3ddeltafunctionorder1(const double *phi, double *delta, BOXP(pg), const double *nx,
+                                     const double *ny, const double *nz, BOXP(gg), BOXP(fb), const double *dx,
+                                     const double *dy, const double *dz) {
+  orc_arr P = VIEW(phi, pg), D = VIEW(delta, pg), NX = VIEW(nx, gg), NY = VIEW(ny, gg);
+  double ix = 1.0 / *dx, iy = 1.0 / *dy;
+  obox b = BOX(fb);
+  FOR_FB(b) {
+    double c = AT(P, i, j, k), x = AT(NX, i, j, k), y = AT(NY, i, j, k);
+    double xp = delta_side(AT(P, i + 1, j, k), c, x, ix), xm = delta_side(AT(P, i - 1, j, k), c, x, ix);
+    double yp = delta_side(AT(P, i, j + 1, k), c, y, iy), ym = delta_side(AT(P, i, j - 1, k), c, y, iy);
+    double zp = delta_side(AT(P, i, j, k + 1), c, x, ix), zm = delta_side(AT(P, i, j, k - 1), c, x, ix);
+    AT(D, i, j, k) = fabs(xp) + fabs(xm) + fabs(yp) + fabs(ym) + fabs(zp) + fabs(zm);
+  }
+}

@@ -67,7 +67,7 @@ class Case:
         self.arrays = {}
         self.scalars = {"dx": 0.1, "dy": 0.125, "dz": 0.15, "dt": 0.01, "cfl": 0.5, "eps": 0.3, "b": 0.7,
                         "vn": 0.8, "use_phi0": 0, "idx": 0, "x": 0.1, "width": 0.3, "width_inner": 0.15,
-                        "beta": 0.2, "gamma": 0.4}
+                        "beta": 0.2, "gamma": 0.4, "cvs": 1}
 
     def box(self, name):
         return self.boxes.get(name, self.gb)

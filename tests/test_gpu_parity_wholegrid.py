@@ -4,12 +4,17 @@ import numpy as np
 import pytest
 
 import harness as H
-from test_oracle_vs_reference import BCS, QUIRKY_BOUNDS, WHOLE_GRID, _prepare
+from test_oracle_vs_reference import BCS, CONTROL_VOLUME, QUIRKY_BOUNDS, UTILS, WHOLE_GRID, _prepare
 
 pytestmark = pytest.mark.gpu
 
 SUMS = {"lsm3dvolumeintegralphilessthanzero_", "lsm3dvolumeintegralphigreaterthanzero_", "lsm3dsurfaceintegral_",
-        "lsm3dvolumeregionphilessthanzero_", "lsm3dvolumeregionphigreaterthanzero_", "lsm3dsurfaceareazerolevelset_"}
+        "lsm3dvolumeregionphilessthanzero_", "lsm3dvolumeregionphigreaterthanzero_", "lsm3dsurfaceareazerolevelset_",
+        "lsm3daveabsdiff_", "lsm3dsurfaceareazerolevelsetdelta_",
+        "lsm3dvolumeintegralphilessthanzerocontrolvolume_", "lsm3dvolumeintegralphigreaterthanzerocontrolvolume_",
+        "lsm3dsurfaceintegralcontrolvolume_", "lsm3dvolumeregionphilessthanzerocontrolvolume_",
+        "lsm3dvolumeregionphigreaterthanzerocontrolvolume_", "lsm3dsurfaceareazerolevelsetcontrolvolume_",
+        "lsm3dsurfaceareazerolevelsetdeltacontrolvolume_"}
 
 
 @pytest.fixture(scope="module")
@@ -40,8 +45,42 @@ def test_whole_grid_bitwise(hip, sym, seed):
     _cmp(hip, sym, c)
 
 
+@pytest.mark.parametrize("sym", UTILS)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_utils(hip, sym, seed):
+    """Average difference, voxel counts, the area from a delta function (sums
+    to 1e-13, counts exactly), the Hessian, the phi-upwind gradient and the
+    first-order delta function (bitwise) against the oracle."""
+    c = H.grid_case(seed)
+    _prepare(c, sym)
+    _cmp(hip, sym, c)
+
+
+@pytest.mark.parametrize("sym", CONTROL_VOLUME)
+@pytest.mark.parametrize("sgn", [1, -1])
+def test_control_volume(hip, sym, sgn):
+    c = H.grid_case(5)
+    c.scalars["cvs"] = sgn
+    _prepare(c, sym)
+    _cmp(hip, sym, c)
+
+
+def test_control_volume_with_no_cell(hip):
+    """No cell in the control volume: the CFL steps come out negative (the
+    reference's initial maximum -1), the max-norm difference is the first
+    cell's, sums and counts are zero -- as the oracle's."""
+    for sym in CONTROL_VOLUME:
+        c = H.grid_case(6)
+        _prepare(c, sym)
+        c.arrays["cv"] = np.full(H.shape_of(c.box("cv")), -1.0)
+        c.scalars["cvs"] = 1
+        _cmp(hip, sym, c)
+
+
 REDUCTIONS = sorted(SUMS) + ["lsm3dmaxnormdiff_", "lsm3dcomputestableadvectiondt_",
-                             "lsm3dcomputestablenormalveldt_", "lsm3dcomputestableconstnormalveldt_"]
+                             "lsm3dcomputestablenormalveldt_", "lsm3dcomputestableconstnormalveldt_",
+                             "lsm3dvoxelcountgreaterthanzero_", "lsm3dvoxelcountlessthanzerocontrolvolume_",
+                             "lsm3dmaxnormdiffcontrolvolume_", "lsm3dcomputestablenormalveldtcontrolvolume_"]
 
 
 @pytest.mark.parametrize("sym", REDUCTIONS)

@@ -81,8 +81,12 @@ def _device_positions(sym):
     """Device arrays the call dereferences.  The whole-grid derivatives form
     their differences in registers and never touch the caller's D1/D2/D3
     scratch, so those may be anything, NULL included.  phi0 of the
-    reinitialization is read only under its flag (the next test)."""
+    reinitialization is read only under its flag (the next test).  The
+    first-order delta function never reads norm_phi_z (its z terms use
+    norm_phi_x, as the reference's do)."""
     unused = ("phi0",) + (("D1", "D2", "D3") if not sym.endswith("local_") else ())
+    if sym == "lsm3ddeltafunctionorder1_":
+        unused += ("nz",)
     return [q for q, (kind, name) in enumerate(capi.SPECS[sym]) if kind in ("R", "IA", "M") and name not in unused]
 
 

@@ -25,6 +25,22 @@ WHOLE_GRID = [
     "lsm3dvolumeregionphilessthanzero_", "lsm3dvolumeregionphigreaterthanzero_",
     "lsm3dsurfaceareazerolevelset_",
 ]
+# Averages, counts, the delta-based area, the Hessian, the phi-upwind gradient
+# and the first-order delta function, and the control-volume forms.
+UTILS = [
+    "lsm3daveabsdiff_", "lsm3dvoxelcountgreaterthanzero_", "lsm3dvoxelcountlessthanzero_",
+    "lsm3dsurfaceareazerolevelsetdelta_", "lsm3dcentralhessian_", "lsm3dphiupwindgradf_",
+    "lsm3ddeltafunctionorder1_",
+]
+CONTROL_VOLUME = [
+    "lsm3dmaxnormdiffcontrolvolume_", "lsm3dcomputestableadvectiondtcontrolvolume_",
+    "lsm3dcomputestablenormalveldtcontrolvolume_", "lsm3dcomputestableconstnormalveldtcontrolvolume_",
+    "lsm3dvolumeintegralphilessthanzerocontrolvolume_", "lsm3dvolumeintegralphigreaterthanzerocontrolvolume_",
+    "lsm3dsurfaceintegralcontrolvolume_", "lsm3dvoxelcountgreaterthanzerocontrolvolume_",
+    "lsm3dvoxelcountlessthanzerocontrolvolume_", "lsm3dvolumeregionphilessthanzerocontrolvolume_",
+    "lsm3dvolumeregionphigreaterthanzerocontrolvolume_", "lsm3dsurfaceareazerolevelsetcontrolvolume_",
+    "lsm3dsurfaceareazerolevelsetdeltacontrolvolume_",
+]
 # Kernels either side of the curvature / reinitialization path: normals,
 # curvatures, orthogonalization and field extension.
 NEIGHBOURS = [
@@ -47,7 +63,7 @@ def test_oracle_exports_everything():
 
 
 def _prepare(case, sym):
-    names = [n for _, n in capi.SPECS[sym]]
+    names = [n for k, n in capi.SPECS[sym] if k in ("R", "M")]
     boxes = H.box_of_array(sym)
     # phi exactly +-zero_tol, where strict and non-strict comparisons differ
     for n in ("phi", "phi0"):
@@ -65,7 +81,13 @@ def _prepare(case, sym):
         g.reshape(-1)[::17] = 0.0
     if sym.startswith("lsm3dcomputestableconstnormalveldt"):
         case.scalars["vn"] = -0.75
-
+    # control volumes with exact zeros, where the counts' >= / <= and the
+    # others' > / < differ
+    if "cv" in names:
+        case.array("cv", boxes["cv"]).reshape(-1)[::7] = 0.0
+    # delta functions with exact zeros and negatives (only positive values count)
+    if "dphi" in names:
+        case.array("dphi", boxes["dphi"]).reshape(-1)[::5] = 0.0
 
 def prepare_neighbours(case, sym):
     """Inputs for NEIGHBOURS that take every branch: gradients exactly zero in
@@ -104,6 +126,38 @@ def test_whole_grid_bitwise(sym, seed):
     c = H.grid_case(seed)
     _prepare(c, sym)
     H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("sym", UTILS)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_utils_bitwise(sym, seed):
+    c = H.grid_case(seed)
+    _prepare(c, sym)
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+@pytest.mark.parametrize("sym", CONTROL_VOLUME)
+@pytest.mark.parametrize("sgn", [1, -1, 0])
+def test_control_volume_bitwise(sym, sgn):
+    c = H.grid_case(5)
+    c.scalars["cvs"] = sgn
+    _prepare(c, sym)
+    H.assert_same(H.run(H.reference(), sym, c), H.run(H.oracle(), sym, c), sym)
+
+
+def test_control_volume_with_no_cell():
+    """No cell in the control volume: the CFL steps divide by the initial
+    maximum -1 (a negative step), the max-norm difference keeps the first
+    cell's value, the sums and counts are zero."""
+    for sym in CONTROL_VOLUME:
+        c = H.grid_case(6)
+        _prepare(c, sym)
+        c.arrays["cv"] = np.full(H.shape_of(c.box("cv")), -1.0)
+        c.scalars["cvs"] = 1
+        a = H.run(H.reference(), sym, c)
+        H.assert_same(a, H.run(H.oracle(), sym, c), sym)
+        if "dt" in sym:
+            assert a[1] < 0, sym
 
 
 @pytest.mark.parametrize("sym", ["lsm3daddconstnormalveltermtolserhs_", "lsm3dcomputereinitializationeqnrhs_"])
