@@ -602,6 +602,9 @@ def main():
                          "report the FP32 field's L-inf / L1 and volume error against FP64")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one device copy of 1 GiB before timing (the known-size kernel for HBM counters)")
+    ap.add_argument("--f32-kernels", type=int, choices=(1, 2), default=1,
+                    help="FP32 fused-stage kernel set: 1 (default) a WENO5 window per cell, 2 the terms of each "
+                         "position shared between cells (lsmhip_set_fused_f32_kernels)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the level-set path runs on a single MI355X here and is not sharded "
@@ -609,8 +612,10 @@ def main():
         sys.exit(2)
 
     import torch
+    import lsmlib_amd
     from lsmlib_amd.solver import DeformationRun
 
+    lsmlib_amd.load("f32").fn("lsmhip_set_fused_f32_kernels", [ctypes.c_int])(a.f32_kernels)
     if a.calibrate_copy:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
