@@ -576,6 +576,46 @@ def band_table(n, steps, warmup):
     return rows, acc
 
 
+def band_traffic_floor(n, line_sizes=(128, 64, 32)):
+    """Host-only bound on the HBM bytes of one fused band stage (FP32) on the
+    narrow-band workload's band at n^3: the bytes of every distinct cache line
+    (or sector) of line_size bytes that the stage touches -- phi_stage on the
+    19-point cross, the marks of the 18 neighbours and the point, phi_n, the
+    three index lists and the output -- as if each were fetched exactly once.
+    The band is level 0 = |phi| < 6 dx and levels 1..3 grown from it along the
+    axes, the band NarrowBandRun builds."""
+    from lsmlib_amd.solver import Grid, phi_spheres_min
+    g = Grid((n, n, n))
+    phi, _ = phi_spheres_min(g, 3, 64, (0.1, 0.9), (0.03, 0.1))
+    lvl0 = np.abs(phi) < 6.0 / n
+    del phi
+    p0 = np.flatnonzero(lvl0.reshape(-1))
+    del lvl0
+    npts = int(p0.size)
+    sy, sz = g.shape[2], g.shape[1] * g.shape[2]
+    alg = 25 * npts
+
+    def distinct(offsets, elem, line):
+        return int(np.unique((offsets * elem) // line).size) * line
+
+    out = {"points": npts, "algorithmic_bytes": alg, "floors": {}}
+    for line in line_sizes:
+        cross = np.concatenate([p0] + [p0 + s * st for st in (1, sy, sz) for s in (-3, -2, -1, 1, 2, 3)])
+        parts = {"phi_stage": distinct(cross, 4, line)}
+        del cross
+        marks = np.concatenate([p0] + [p0 + s * st for st in (1, sy, sz) for s in (-2, -1, 1, 2, 3)])
+        parts["marks"] = distinct(marks, 1, line)
+        del marks
+        parts["phi_n"] = distinct(p0, 4, line)
+        parts["index_lists"] = 3 * distinct(np.arange(npts), 4, line)
+        parts["output"] = distinct(p0, 4, line)
+        total = sum(parts.values())
+        out["floors"][str(line)] = {"bytes": total, "ratio": round(total / alg, 3), "parts": parts}
+        print("[bench] %d-byte lines: %.1f MB, %.2fx algorithmic" % (line, total / 1e6, total / alg),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -597,6 +637,8 @@ def main():
     ap.add_argument("--band", action="store_true",
                     help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
                          "instead of the whole-grid step")
+    ap.add_argument("--band-floor", action="store_true",
+                    help="host only: the cache-line bound on one fused band stage's HBM bytes at n^3 (no GPU)")
     ap.add_argument("--precision-study", action="store_true",
                     help="integrate the deformation workload over t in [0, 3] in FP64 and FP32 (fused stages) and "
                          "report the FP32 field's L-inf / L1 and volume error against FP64")
@@ -610,6 +652,12 @@ def main():
         print("bench.py measures one GPU: the level-set path runs on a single MI355X here and is not sharded "
               "across GPUs; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if a.band_floor:
+        res = band_traffic_floor(a.n)
+        print(json.dumps({"metric": "fused band stage HBM bytes floor @%d^3 (FP32)" % a.n, "data": "synthetic",
+                          **res}))
+        return
 
     import torch
     import lsmlib_amd
