@@ -576,6 +576,37 @@ def band_table(n, steps, warmup):
     return rows, acc
 
 
+def adjoint_table(n, steps):
+    """Wall time of the shape-matching loss (forward only) and of the loss with
+    its gradients (forward with checkpoints every 20 steps, then the reverse
+    sweep that recomputes each segment and walks its stages back), n^3, FP32
+    and FP64, synchronised around each call."""
+    import torch
+    from lsmlib_amd.adjoint import ShapeMatching
+    out = {}
+    for prec in ("f32", "f64"):
+        sm = ShapeMatching(n=n, precision=prec, steps=steps)
+        sm.value(sm.a, sm.phi0)  # warm-up: loads the kernels, allocates the fields
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = sm.value(sm.a, sm.phi0)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss2, ga, gphi = sm.value_and_grads()
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        sm.engine.solver.lib.check()
+        out[prec] = {"steps": steps, "forward_ms": round(1e3 * tf, 2), "loss_and_gradients_ms": round(1e3 * tb, 2),
+                     "gradient_over_forward": round(tb / tf, 2), "loss": loss2,
+                     "grad_speed_l2": float(torch.linalg.vector_norm(ga.double()))}
+        print("[bench] adjoint %s %d^3 x %d steps: forward %.1f ms, loss + gradients %.1f ms" % (
+            prec, n, steps, 1e3 * tf, 1e3 * tb), file=sys.stderr, flush=True)
+        del sm, ga, gphi
+        torch.cuda.empty_cache()
+    return out
+
+
 def band_traffic_floor(n, line_sizes=(128, 64, 32)):
     """Host-only bound on the HBM bytes of one fused band stage (FP32) on the
     narrow-band workload's band at n^3: the bytes of every distinct cache line
@@ -637,6 +668,9 @@ def main():
     ap.add_argument("--band", action="store_true",
                     help="time the narrow-band workload (band construction and local steps, FP64 and FP32) at n^3 "
                          "instead of the whole-grid step")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the shape-matching loss and its gradients (adjoint of the normal-motion stages) at "
+                         "n^3 over --steps steps (default n 256, steps 200)")
     ap.add_argument("--band-floor", action="store_true",
                     help="host only: the cache-line bound on one fused band stage's HBM bytes at n^3 (no GPU)")
     ap.add_argument("--precision-study", action="store_true",
@@ -693,6 +727,15 @@ def main():
         print(json.dumps({"metric": "narrow-band local step @%d^3" % a.n, "unit": "band cell-updates/s",
                           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "data": "synthetic", "runs": rows,
                           "accuracy_fp32_vs_fp64": acc}))
+        return
+    if a.adjoint:
+        n = a.n if a.n != 512 else 256
+        steps = a.steps if a.steps != 40 else 200
+        res = adjoint_table(n, steps)
+        print(json.dumps({"metric": "adjoint of the level-set evolution @%d^3" % n, "unit": "ms", "n_gpus": 1,
+                          "data": "synthetic", "config": {"workload": "shape matching: a sphere r=0.2 under "
+                                                          "V = 1 + 0.3 tanh(g), RK3 normal motion, periodic, "
+                                                          "checkpoints every 20 steps"}, "runs": res}))
         return
     if a.precision_study:
         from lsmlib_amd.solver import precision_study

@@ -176,6 +176,49 @@ def test_c3_band_fused_f32_within_tolerance():
     assert max(abs(a / b - 1) for a, b in zip(r32.dts, r64.dts)) < 1e-5
 
 
+def test_c3_band_fused_f32_at_512_within_tolerance():
+    """The narrow-band workload at full size, 512^3 x 200 steps with the band
+    rebuilt every 10: the FP32 fused band step against the FP64 fused one
+    (bitwise the FP64 drop-in chain and the reference kernels' golden,
+    test_c3_band_full_is_bitwise_the_reference) from the same initial field,
+    at the level-0 points of both last bands.
+
+    The cut-off of the speed vanishes at |phi| = gamma = 6 dx, the band's
+    edge, and grows quadratically inside it, so a point starting just inside
+    the edge leaves it at a time set by how close it starts: points whose
+    initial |phi| is within 0.1 % of gamma are released at different steps in
+    FP32 and FP64 and end up to about a cell apart (measured 2.6e-3 = 1.3 dx
+    at 61 of 10.0M points).  The bounds: mean difference below 1e-7
+    (measured 3.3e-8); fewer than 1e-4 of the points apart by more than 1e-5
+    (4.5e-5); below 5e-5 everywhere more than 3 cells from those edge points
+    (1.9e-5 over 7.4M points); below 2 dx everywhere; every dt within 1e-5."""
+    from scipy import ndimage
+    from lsmlib_amd.solver import Grid, NarrowBandRun, phi_spheres_min
+    n = 512
+    h = 1.0 / n
+    phi0, _ = phi_spheres_min(Grid((n, n, n)), 3, 64, (0.1, 0.9), (0.03, 0.1))
+    r64 = NarrowBandRun(n=n, steps=200, rebuild_every=10, phi0=phi0, fused=True)
+    p64 = r64.solver.host(r64.run())
+    nb64, dts64 = r64.nb.cpu().numpy(), list(r64.dts)
+    del r64
+    r32 = NarrowBandRun(n=n, steps=200, rebuild_every=10, phi0=phi0, precision="f32", fused=True)
+    p32 = r32.solver.host(r32.run())
+    lv0 = (nb64 == 1) & (r32.nb.cpu().numpy() == 1)
+    d = np.abs(p64 - p32)
+    dl = d[lv0]
+    edge = np.abs(np.abs(phi0) / (6 * h) - 1.0) < 1e-3
+    near = ndimage.binary_dilation(edge, structure=np.ones((3, 3, 3), bool), iterations=3)
+    away = d[lv0 & ~near]
+    print("512^3 x 200: FP32 fused band vs FP64: L-inf %.3e, mean %.3e, %d of %d points above 1e-5; "
+          "more than 3 cells from the edge points: L-inf %.3e over %d" % (
+              dl.max(), dl.mean(), int((dl > 1e-5).sum()), dl.size, away.max(), away.size))
+    assert dl.mean() < 1e-7
+    assert (dl > 1e-5).sum() < 1e-4 * dl.size
+    assert away.max() < 5e-5 and away.size > 0.5 * dl.size
+    assert dl.max() < 2 * h
+    assert max(abs(a / b - 1) for a, b in zip(r32.dts, dts64)) < 1e-5
+
+
 def test_fused_band_stage_reports_a_stencil_outside_the_band():
     """A band grown to one level only: WENO5 stencils of level-0 points reach
     cells outside the band (mark 0), where the drop-in chain would read stale
