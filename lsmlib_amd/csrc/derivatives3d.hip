@@ -22,63 +22,46 @@ struct Axes3 {
   View<real_t> o[3];
 };
 
-// Loads phi at offsets -R..R along axis d (f[s+3] holds offset s).
-template <int R>
-__device__ __forceinline__ void load_line(const View<const real_t>& phi, int i, int j, int k, int d, real_t f[7]) {
-  const int di = d == 0, dj = d == 1, dk = d == 2;
-#pragma unroll
-  for (int s = -R; s <= R; s++) f[s + 3] = phi(i + s * di, j + s * dj, k + s * dk);
-}
-
+// HJ-ENO1/2/3 and HJ-WENO5 plus and minus derivatives, and their velocity-
+// upwinded forms, on the plane march: the lines of phi come from LDS (x, y)
+// and the thread's carried column (z); the arithmetic is stencil.hpp's, in the
+// reference's operation order.
 template <int ORDER>
-__global__ void __launch_bounds__(256) hj_plus_minus_kernel(Box fb, Axes3 plus, Axes3 minus,
-                                                            View<const real_t> phi, real_t ix, real_t iy,
-                                                            real_t iz) {
-  LSMHIP_CELL(fb);
-  const real_t inv[3] = {ix, iy, iz};
-  // All three lines are loaded before the first store: an output may alias phi
-  // as far as the compiler knows, so a load after a store would wait for it.
-  // (One cell per thread: a pair of cells along k measured slower here, the
-  // registers of two full crosses costing more occupancy than they gain.)
-  real_t f[3][7];
+struct PlusMinusLine {
+  Axes3 plus, minus;
+  real_t inv[3];
+  struct In {};
+  __device__ In load(int, int, int) const { return In{}; }
+  __device__ void store(int i, int j, int k, const real_t (&fx)[7], const real_t (&fy)[7], const real_t (&fz)[7],
+                        const In&) const {
+    const real_t* f[3] = {fx, fy, fz};
 #pragma unroll
-  for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k, d, f[d]);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    real_t p, m;
-    stencil::plus_minus<ORDER>(f[d], inv[d], p, m);
-    plus.o[d](i, j, k) = p;
-    minus.o[d](i, j, k) = m;
-  }
-}
-
-template <int ORDER>
-__global__ void __launch_bounds__(256) hj_upwind_kernel(Box fb, Axes3 out, View<const real_t> phi,
-                                                        View<const real_t> vx, View<const real_t> vy,
-                                                        View<const real_t> vz, real_t ix, real_t iy, real_t iz) {
-  LSMHIP_CELL_PAIR(fb);
-  const real_t inv[3] = {ix, iy, iz};
-  // Every line of both cells is loaded (inside phi's checked box) before the
-  // first store, also where the speed below zero_tol leaves it unused.
-  real_t vel[2][3], f[2][3][7];
-#pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk) {
-      vel[c][0] = vx(i, j, k0 + c);
-      vel[c][1] = vy(i, j, k0 + c);
-      vel[c][2] = vz(i, j, k0 + c);
-#pragma unroll
-      for (int d = 0; d < 3; d++) load_line<(ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3))>(phi, i, j, k0 + c, d, f[c][d]);
+    for (int d = 0; d < 3; d++) {
+      real_t p, m;
+      stencil::plus_minus<ORDER>(f[d], inv[d], p, m);
+      plus.o[d](i, j, k) = p;
+      minus.o[d](i, j, k) = m;
     }
+  }
+};
+template <int ORDER>
+struct UpwindLine {
+  Axes3 out;
+  View<const real_t> vx, vy, vz;
+  real_t inv[3];
+  struct In {
+    real_t v[3];
+  };
+  __device__ In load(int i, int j, int k) const { return In{{vx(i, j, k), vy(i, j, k), vz(i, j, k)}}; }
+  __device__ void store(int i, int j, int k, const real_t (&fx)[7], const real_t (&fy)[7], const real_t (&fz)[7],
+                        const In& in) const {
+    const real_t* f[3] = {fx, fy, fz};
 #pragma unroll
-  for (int c = 0; c < 2; c++)
-    if (c < nk)
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-        out.o[d](i, j, k0 + c) = fabs(vel[c][d]) < kZeroTol
-                                     ? real_t(0)
-                                     : stencil::upwind<ORDER>(f[c][d], inv[d], vel[c][d] > real_t(0));
-}
+    for (int d = 0; d < 3; d++)
+      out.o[d](i, j, k) =
+          fabs(in.v[d]) < kZeroTol ? real_t(0) : stencil::upwind<ORDER>(f[d], inv[d], in.v[d] > real_t(0));
+  }
+};
 
 // Second-order central gradient and Laplacian on the plane march
 // (src/toolbox/lsm_spatial_derivatives3d.f.in, lsm3dCentralGradOrder2 and
@@ -102,17 +85,21 @@ struct Laplacian2Star {
   }
 };
 
-__global__ void __launch_bounds__(256) central4_kernel(Box fb, Axes3 out, View<const real_t> phi, real_t fx,
-                                                       real_t fy, real_t fz) {
-  LSMHIP_CELL(fb);
-  const real_t e = real_t(8);
-  real_t f[3][7];  // loads first: a load after a store to a possibly aliasing output would wait for it
+// Fourth-order central gradient on the plane march (lsm3dCentralGradOrder4),
+// in the reference's operation order.
+struct Central4Line {
+  Axes3 out;
+  real_t r[3];
+  struct In {};
+  __device__ In load(int, int, int) const { return In{}; }
+  __device__ void store(int i, int j, int k, const real_t (&fx)[7], const real_t (&fy)[7], const real_t (&fz)[7],
+                        const In&) const {
+    const real_t e = real_t(8);
+    const real_t* f[3] = {fx, fy, fz};
 #pragma unroll
-  for (int d = 0; d < 3; d++) load_line<2>(phi, i, j, k, d, f[d]);
-  const real_t r[3] = {fx, fy, fz};
-#pragma unroll
-  for (int d = 0; d < 3; d++) out.o[d](i, j, k) = (-f[d][5] + e * f[d][4] + f[d][1] - e * f[d][2]) * r[d];
-}
+    for (int d = 0; d < 3; d++) out.o[d](i, j, k) = (-f[d][5] + e * f[d][4] + f[d][1] - e * f[d][2]) * r[d];
+  }
+};
 
 // lsm3dCentralHessian (src/toolbox/lsm_spatial_derivatives3d.f.in:3658-3715):
 // compact second differences over dx*dx etc., mixed ones as 0.25 (f(+,+) -
@@ -171,7 +158,7 @@ __global__ void __launch_bounds__(256) phi_upwind_grad_kernel(Box fb, Axes3 out,
 
 __global__ void __launch_bounds__(256) average_grad_kernel(Box fb, Axes3 out, Axes3 plus, Axes3 minus) {
   LSMHIP_CELL(fb);
-  real_t p[3], m[3];  // loads first, as in central4_kernel
+  real_t p[3], m[3];  // loads first: a load after a store to a possibly aliasing output would wait for it
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     p[d] = plus.o[d](i, j, k);
@@ -198,14 +185,16 @@ void hj_plus_minus(const char* who, real_t* pxp, real_t* pyp, real_t* pzp, const
                    real_t* pzm, const Box& gm, const real_t* phi, const Box& pg, const Box& fb, real_t dx, real_t dy,
                    real_t dz) {
   if (fb.empty()) return;
-  const int w = ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3);
+  constexpr int w = ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3);
   if (!check_box(gp, fb, who, "grad_phi_plus") || !check_box(gm, fb, who, "grad_phi_minus") ||
       !check_box(pg, fb.widen(w, w, w), who, "phi") || !check_size(pg, who) || !check_size(gp, who) ||
       !check_size(gm, who))
     return;
-  hipLaunchKernelGGL(hj_plus_minus_kernel<ORDER>, grid_for(fb), block2d(), 0, stream(), fb,
-                     axes(pxp, pyp, pzp, gp), axes(pxm, pym, pzm, gm), view(phi, pg), real_t(1.0) / dx,
-                     real_t(1.0) / dy, real_t(1.0) / dz);
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL((march::line_march_kernel<w, PlusMinusLine<ORDER>>), march::grid_for(tl), march::block(), 0,
+                     stream(), fb, view(phi, pg), tl,
+                     PlusMinusLine<ORDER>{axes(pxp, pyp, pzp, gp), axes(pxm, pym, pzm, gm),
+                                          {real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz}});
   finish(who);
 }
 
@@ -214,14 +203,16 @@ void hj_upwind(const char* who, real_t* px, real_t* py, real_t* pz, const Box& g
                const real_t* vx, const real_t* vy, const real_t* vz, const Box& vg, const Box& fb, real_t dx,
                real_t dy, real_t dz) {
   if (fb.empty()) return;
-  const int w = ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3);
+  constexpr int w = ORDER == 1 ? 1 : (ORDER == 2 ? 2 : 3);
   if (!check_box(g, fb, who, "grad_phi") || !check_box(vg, fb, who, "vel") ||
       !check_box(pg, fb.widen(w, w, w), who, "phi") || !check_size(pg, who) || !check_size(g, who) ||
       !check_size(vg, who))
     return;
-  hipLaunchKernelGGL(hj_upwind_kernel<ORDER>, grid_for_pair(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g),
-                     view(phi, pg), view(vx, vg), view(vy, vg), view(vz, vg), real_t(1.0) / dx, real_t(1.0) / dy,
-                     real_t(1.0) / dz);
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL((march::line_march_kernel<w, UpwindLine<ORDER>>), march::grid_for(tl), march::block(), 0,
+                     stream(), fb, view(phi, pg), tl,
+                     UpwindLine<ORDER>{axes(px, py, pz, g), view(vx, vg), view(vy, vg), view(vz, vg),
+                                       {real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz}});
   finish(who);
 }
 
@@ -312,8 +303,9 @@ LSMHIP_EXPORT void lsm3dcentralgradorder4_(real_t* px, real_t* py, real_t* pz, L
       !check_size(pg, who))
     return;
   const real_t c = real_t(0.0833333333333333333333);
-  hipLaunchKernelGGL(central4_kernel, grid_for(fb), block2d(), 0, stream(), fb, axes(px, py, pz, g), view(phi, pg),
-                     c / *dx, c / *dy, c / *dz);
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL((march::line_march_kernel<2, Central4Line>), march::grid_for(tl), march::block(), 0, stream(),
+                     fb, view(phi, pg), tl, Central4Line{axes(px, py, pz, g), {c / *dx, c / *dy, c / *dz}});
   finish(who);
 }
 

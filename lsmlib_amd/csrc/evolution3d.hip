@@ -4,6 +4,7 @@
 // src/toolbox/lsm_tvd_runge_kutta3d.f, src/reinitialization/lsm_reinitialization3d.f.in,
 // src/geometry/lsm_curvature3d.f.in.
 #include "lsmhip_core.hpp"
+#include "march.hpp"
 #include "stencil.hpp"
 
 using namespace lsmhip;
@@ -114,30 +115,12 @@ struct CurvFactors {
 };
 
 // Second-order mean curvature (src/geometry/lsm_curvature3d.f.in:15-128): the inputs at
-// one cell, loaded before any is used.
+// one cell.
 struct CurvInputs {
   real_t c, xp, xm, yp, ym, zp, zm;  // F at the cell and its 6 neighbours
   real_t xy[4], xz[4], yz[4];        // F at (+,+), (+,-), (-,+), (-,-) of each plane pair
   real_t x, y, z, gm;
 };
-__device__ __forceinline__ CurvInputs curv_inputs(const View<const real_t>& F, int i, int j, int k,
-                                                  const View<const real_t>& px, const View<const real_t>& py,
-                                                  const View<const real_t>& pz, const View<const real_t>& gmag) {
-  return CurvInputs{F(i, j, k),
-                    F(i + 1, j, k),
-                    F(i - 1, j, k),
-                    F(i, j + 1, k),
-                    F(i, j - 1, k),
-                    F(i, j, k + 1),
-                    F(i, j, k - 1),
-                    {F(i + 1, j + 1, k), F(i + 1, j - 1, k), F(i - 1, j + 1, k), F(i - 1, j - 1, k)},
-                    {F(i + 1, j, k + 1), F(i + 1, j, k - 1), F(i - 1, j, k + 1), F(i - 1, j, k - 1)},
-                    {F(i, j + 1, k + 1), F(i, j + 1, k - 1), F(i, j - 1, k + 1), F(i, j - 1, k - 1)},
-                    px(i, j, k),
-                    py(i, j, k),
-                    pz(i, j, k),
-                    gmag(i, j, k)};
-}
 // Compact second differences of F, numerator over |grad phi|^3 (zero when that
 // is below zero_tol), in the reference's operation order.
 __device__ __forceinline__ real_t mean_curvature_of(const CurvInputs& a, const CurvFactors& f) {
@@ -152,15 +135,34 @@ __device__ __forceinline__ real_t mean_curvature_of(const CurvInputs& a, const C
   return stencil::curv_num(xx, xy, xz, yy, yz, zz, a.x, a.y, a.z) / den;
 }
 
-__global__ void __launch_bounds__(256) mean_curv_kernel(Box fb, View<real_t> kappa, View<const real_t> phi,
-                                                        View<const real_t> px, View<const real_t> py,
-                                                        View<const real_t> pz, View<const real_t> gmag,
-                                                        CurvFactors f) {
-  LSMHIP_CELLS(fb, kPointNK);
-  pair_cells(
-      k0, nk, [&](int k) { return curv_inputs(phi, i, j, k, px, py, pz, gmag); },
-      [&](int k, const CurvInputs& a) { kappa(i, j, k) = mean_curvature_of(a, f); });
-}
+// The mean curvature on the plane march: the 19-point cross from the march,
+// the gradient and its magnitude loaded per cell.
+struct MeanCurvMarch {
+  View<real_t> kappa;
+  View<const real_t> px, py, pz, gmag;
+  CurvFactors f;
+  struct In {
+    real_t x, y, z, gm;
+  };
+  __device__ In load(int i, int j, int k) const { return In{px(i, j, k), py(i, j, k), pz(i, j, k), gmag(i, j, k)}; }
+  __device__ void store(int i, int j, int k, const march::Cross19& c, const In& v) const {
+    const CurvInputs a{c.c,
+                       c.xp,
+                       c.xm,
+                       c.yp,
+                       c.ym,
+                       c.zp,
+                       c.zm,
+                       {c.xy[0], c.xy[1], c.xy[2], c.xy[3]},
+                       {c.xz[0], c.xz[1], c.xz[2], c.xz[3]},
+                       {c.yz[0], c.yz[1], c.yz[2], c.yz[3]},
+                       v.x,
+                       v.y,
+                       v.z,
+                       v.gm};
+    kappa(i, j, k) = mean_curvature_of(a, f);
+  }
+};
 
 // RK stage kinds: 1 = u + dt L;  2 = 0.5 (u + u1 + dt L);  32 = 0.75 u + 0.25 (u1 + dt L);
 // 33 = 1/3 u + 2/3 (u2 + dt L)  (lsm_tvd_runge_kutta3d.f:78, :240, :403, :490).
@@ -350,8 +352,10 @@ LSMHIP_EXPORT void lsm3dcomputemeancurvatureorder2_(real_t* kappa, LSMHIP_BOXP(k
     return;
   CurvFactors f{real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
                 real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz};
-  hipLaunchKernelGGL(mean_curv_kernel, grid_for_cells(fb, kPointNK), block2d(), 0, stream(), fb, view(kappa, kg), view(phi, pg),
-                     view(px, gg), view(py, gg), view(pz, gg), view(gmag, gg), f);
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL(march::cross19_march_kernel<MeanCurvMarch>, march::grid_for(tl), march::block(), 0, stream(), fb,
+                     view(phi, pg), tl,
+                     MeanCurvMarch{view(kappa, kg), view(px, gg), view(py, gg), view(pz, gg), view(gmag, gg), f});
   finish(who);
 }
 

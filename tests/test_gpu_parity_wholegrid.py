@@ -132,6 +132,23 @@ def test_whole_grid_bitwise_odd_plane_count(hip, sym):
     _cmp(hip, sym, c)
 
 
+PLANE_MARCH = ["lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_",
+               "lsm3dcomputemeancurvatureorder2_",
+               "lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_", "lsm3dupwindhjeno1_",
+               "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_", "lsm3dupwindhjweno5_"]
+
+
+@pytest.mark.parametrize("sym", PLANE_MARCH)
+def test_plane_march_over_many_tiles_and_chunks(hip, sym):
+    """The kernels that march 64 x 8 tiles through chunks of 16 planes, on a
+    fill box of 141 x 21 x 38 cells: three tiles along i (the last one 13
+    wide), three along j (the last 5), three chunks along k (the last 6);
+    phi's ghost box is three cells wider than the fill box on every side."""
+    c = H.grid_case(6, gb=(-3, 143, -3, 23, -3, 40), fb=(0, 140, 0, 20, 0, 37))
+    _prepare(c, sym)
+    _cmp(hip, sym, c)
+
+
 @pytest.mark.parametrize("sym", QUIRKY_BOUNDS)
 def test_quirky_loop_bounds(hip, sym):
     c = H.grid_case(3, mixed_boxes=False)
