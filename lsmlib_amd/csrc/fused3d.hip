@@ -7,11 +7,12 @@
 // Memory layout and traffic: phi_stage, phi_n and the output share one ghost
 // box.  A block owns a TX x TY tile of the (i,j) plane and marches along k
 // through a chunk of KC planes.  Each thread keeps the seven values of its own
-// column around plane k (the WENO5 reach is +-3) in registers and shifts them
-// as k advances, so the z stencil needs neither LDS nor index arithmetic; the
-// current plane with its 3-cell x/y halo is staged in LDS for the x and y
-// stencils (interior from the threads' windows, the halo loaded by one thread
-// per halo cell).  Each plane is read from global memory once per block;
+// column around plane k (the WENO5 reach is +-3) in registers (in the
+// advection and normal-motion kernel with the next plane's value in flight),
+// so the z stencil needs neither LDS nor index arithmetic; the current plane
+// with its 3-cell x/y halo is staged in LDS for the x and y stencils
+// (interior from the threads' windows, the halo loaded by one thread per halo
+// cell).  Each plane is read from global memory once per block;
 // neighbouring tiles share halos through L2: blocks are numbered so that each
 // XCD (blocks are dealt round-robin to the 8 XCDs) processes one contiguous
 // range of tiles.
@@ -71,7 +72,7 @@ __device__ __forceinline__ double weno_up(const double f[7], double inv, bool vp
   return stencil::upwind<5>(f, inv, vpos);
 }
 __device__ __forceinline__ float weno_up(const float f[7], float inv, bool vpos) {
-  return stencil::fastf::upwind(f, inv, vpos);
+  return stencil::fastf::upwind_wave(f, inv, vpos);
 }
 
 // Square root and reinitialization RHS of the fused stages: the reference's
@@ -149,10 +150,6 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
   const real_t* us = a.us;
   const long long sz = a.sz;
 
-  // z window of the own column: zw[s+3] = u(i, j, k+s)
-  real_t zw[7];
-#pragma unroll
-  for (int s = 0; s < 6; s++) zw[s] = own_in ? us[own + sz * (kb - H + s - gb.klo)] : real_t(0);
   real_t ta0 = 0, tb0 = 0, ta1 = 0, tb1 = 0;
   if (RHS == kAdvectDeform && active) {
     ta0 = a.ta[0][i - gb.ilo];
@@ -165,27 +162,37 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
   const real_t X0 = real_t(2) * a.tfac * a.ix * ta0 * tb1, X1 = -(a.tfac * a.iy) * tb0 * ta1,
                X2 = -(a.tfac * a.iz) * tb0 * tb1;
 
-  for (int k = kb; k <= ke; k++) {
+  // plane k + d of the own column
+  auto zload = [&](int k, int d) { return own_in ? us[own + sz * (k + d - gb.klo)] : real_t(0); };
+  // the halo element and the per-cell inputs of plane k
+  struct In {
+    real_t hv, uc, c0, c1, c2;
+  };
+  auto load_in = [&](int k) {
     const long long zoff = sz * (k - gb.klo);
-    zw[6] = own_in ? us[own + zoff + H * sz] : real_t(0);
-    const real_t hv = hoff >= 0 ? us[hoff + zoff] : real_t(0);
-    // per-cell inputs of this plane, issued before the stencil work so that their latency is hidden
     const long long off = own + zoff;
-    const real_t ucv = STAGE != 1 && active ? a.uc[off] : real_t(0);
-    real_t c0 = real_t(0), c1 = real_t(0), c2 = real_t(0);
+    In v{real_t(0), real_t(0), real_t(0), real_t(0), real_t(0)};
+    if (hoff >= 0) v.hv = us[hoff + zoff];
     if (active) {
+      if (STAGE != 1) v.uc = a.uc[off];
       if (RHS == kAdvectField) {
-        c0 = a.vx[off];
-        c1 = a.vy[off];
-        c2 = a.vz[off];
+        v.c0 = a.vx[off];
+        v.c1 = a.vy[off];
+        v.c2 = a.vz[off];
       } else if (RHS == kReinit) {
-        c0 = a.use_phi0 ? a.phi0[off] : real_t(0);
+        v.c0 = a.use_phi0 ? a.phi0[off] : real_t(0);
       } else if (RHS == kNormalField) {
-        c0 = a.vnf[off];
+        v.c0 = a.vnf[off];
       }
     }
+    return v;
+  };
+  // cell (i, j, k) with its z window w[s + 3] = u(i, j, k + s) and inputs in
+  auto cell = [&](int k, const real_t (&w)[7], const In& in) {
+    const long long off = own + sz * (k - gb.klo);
+    const real_t hv = in.hv, ucv = in.uc, c0 = in.c0, c1 = in.c1, c2 = in.c2;
     __syncthreads();  // the previous plane's reads are done
-    plane[ly * PX + lx] = zw[3];
+    plane[ly * PX + lx] = w[3];
     if (tid < NHALO) plane[hslot] = hv;
     __syncthreads();
     if (active) {
@@ -194,7 +201,7 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
       for (int s = -H; s <= H; s++) {
         fx[s + 3] = plane[ly * PX + lx + s];
         fy[s + 3] = plane[(ly + s) * PX + lx];
-        fz[s + 3] = zw[s + 3];
+        fz[s + 3] = w[s + 3];
       }
       real_t r;
       if (sizeof(real_t) == 4 && RHS == kAdvectDeform) {
@@ -206,21 +213,21 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
         const real_t pz = fabs(wz) < kZeroTol * a.iz ? real_t(0) : weno_up(fz, real_t(1), wz > real_t(0));
         r = real_t(0) - (ux * px + vy * py + wz * pz);
       } else if (RHS == kAdvectField || RHS == kAdvectDeform) {
-        real_t u, v, w;
+        real_t u, v, vw;
         if (RHS == kAdvectField) {
           u = c0;
           v = c1;
-          w = c2;
+          vw = c2;
         } else {
           const real_t ta2 = a.ta[2][k - gb.klo], tb2 = a.tb[2][k - gb.klo], c = a.tfac;
           u = real_t(2) * ta0 * tb1 * tb2 * c;
           v = -(tb0 * ta1 * tb2 * c);
-          w = -(tb0 * tb1 * ta2 * c);
+          vw = -(tb0 * tb1 * ta2 * c);
         }
         const real_t px = fabs(u) < kZeroTol ? real_t(0) : weno_up(fx, a.ix, u > real_t(0));
         const real_t py = fabs(v) < kZeroTol ? real_t(0) : weno_up(fy, a.iy, v > real_t(0));
-        const real_t pz = fabs(w) < kZeroTol ? real_t(0) : weno_up(fz, a.iz, w > real_t(0));
-        r = real_t(0) - (u * px + v * py + w * pz);
+        const real_t pz = fabs(vw) < kZeroTol ? real_t(0) : weno_up(fz, a.iz, vw > real_t(0));
+        r = real_t(0) - (u * px + v * py + vw * pz);
       } else {
         real_t xp, xm, yp, ym, zp, zm;
         weno_pm(fx, a.ix, xp, xm);
@@ -238,8 +245,44 @@ __global__ void __launch_bounds__(TX* TY) fused_stage_kernel(FusedArgs a) {
       const real_t uc = STAGE == 1 ? fx[3] : ucv;
       a.out[off] = rk_combine<STAGE>(uc, fx[3], a.dt, r);
     }
+  };
+
+  // The z window, one plane ahead: plane k + 4 of the own column (and the
+  // halo and inputs of plane k + 1) load while plane k is worked on.  FP32
+  // keeps the window in a ring of eight registers: in a group of eight planes
+  // starting at k, plane k + q has u(i, j, k + q + s) in z[(q + s + 3) % 8]
+  // for s = -3 .. 4, and the group is unrolled, so the window advances
+  // without moving registers.  The last planes of the chunk (fewer than
+  // eight), and every plane in FP64 (where the unrolled group takes registers
+  // enough to cost a resident block per CU), shift the window instead.
+  real_t z[8];
 #pragma unroll
-    for (int s = 0; s < 6; s++) zw[s] = zw[s + 1];
+  for (int s = 0; s < 7; s++) z[s] = zload(kb, s - H);
+  In cur = load_in(kb);
+  int k = kb;
+  if constexpr (sizeof(real_t) == 4) {
+    for (; k + 7 <= ke; k += 8) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const bool more = k + q < ke;
+        if (more) z[(q + 7) % 8] = zload(k + q, H + 1);
+        const In nxt = more ? load_in(k + q + 1) : In{};
+        const real_t w[7] = {z[q % 8],       z[(q + 1) % 8], z[(q + 2) % 8], z[(q + 3) % 8],
+                             z[(q + 4) % 8], z[(q + 5) % 8], z[(q + 6) % 8]};
+        cell(k + q, w, cur);
+        cur = nxt;
+      }
+    }
+  }
+  for (; k <= ke; k++) {
+    const bool more = k < ke;
+    if (more) z[7] = zload(k, H + 1);
+    const In nxt = more ? load_in(k + 1) : In{};
+    const real_t w[7] = {z[0], z[1], z[2], z[3], z[4], z[5], z[6]};
+    cell(k, w, cur);
+    cur = nxt;
+#pragma unroll
+    for (int s = 0; s < 7; s++) z[s] = z[s + 1];
   }
 }
 

@@ -353,6 +353,30 @@ __device__ __forceinline__ float upwind(const float f[7], float inv, bool vpos) 
   return weno5(a, b, c, e, g) * inv;
 }
 
+// upwind() where the side taken is usually the same across a wave (a smooth
+// velocity): a wave whose lanes all take one side forms that side's window
+// alone, without a per-lane choice of operands.  Each lane's arithmetic is
+// that of upwind(), so a cell's result does not depend on the other lanes.
+__device__ __forceinline__ float upwind_wave(const float f[7], float inv, bool vpos) {
+  const unsigned long long pos = __builtin_amdgcn_ballot_w64(vpos), live = __builtin_amdgcn_ballot_w64(true);
+  float d[6];
+#pragma unroll
+  for (int s = -2; s <= 3; s++) d[s + 2] = f[s + 3] - f[s + 2];
+  // (the empty asm keeps each side's arithmetic whole in its own branch; merged
+  // tails would need register moves on every path)
+  float r;
+  if (pos == live) {
+    r = fwd(win5(d[0], d[1], d[2], d[3], d[4]), d[1], d[2], d[3]) * inv;
+    asm volatile("" : "+v"(r));
+  } else if (pos == 0) {
+    r = fwd(win5(d[5], d[4], d[3], d[2], d[1]), d[4], d[3], d[2]) * inv;
+    asm volatile("" : "+v"(r));
+  } else {
+    r = upwind(f, inv, vpos);
+  }
+  return r;
+}
+
 // Reinitialization RHS of one cell (the reference's formula,
 // lsm_reinitialization3d.f.in:167-199 and :251) with the hardware square root
 // and reciprocal square root (about 1 ulp each) in place of the correctly
