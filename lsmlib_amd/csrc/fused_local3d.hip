@@ -43,35 +43,53 @@ struct LocalStage {
 };
 
 // The 19-point cross of phi around (i, j, k) (offset p) and the marks of the
-// 18 neighbours whose differences the stencil reads, all loads issued before
-// any is tested.  Returns false (and flags) when the cross leaves gb or the
-// band's box, or reaches a cell outside the band.
-__device__ __forceinline__ bool gather_cross(const LocalStage& a, int i, int j, int k, long long p, real_t f[3][7],
-                                             signed char m[3][6], int* flag) {
+// 18 neighbours whose differences the stencil reads.  cross_in_boxes: the
+// cross lies in gb and the band's box (no memory read); load_cross issues the
+// loads without testing them; cross_in_band: no neighbour lies outside the
+// band (mark 0).
+__device__ __forceinline__ bool cross_in_boxes(const LocalStage& a, int i, int j, int k) {
   const Box& nbb = a.fbm.b;
-  if (!a.gb.has(i - 3, j - 3, k - 3) || !a.gb.has(i + 3, j + 3, k + 3) || !nbb.has(i - 2, j - 2, k - 2) ||
-      !nbb.has(i + 3, j + 3, k + 3)) {
-    *flag = 1;
-    return false;
-  }
-  const long long st[3] = {1, a.sy, a.sz};
+  return a.gb.has(i - 3, j - 3, k - 3) && a.gb.has(i + 3, j + 3, k + 3) && nbb.has(i - 2, j - 2, k - 2) &&
+         nbb.has(i + 3, j + 3, k + 3);
+}
+__device__ __forceinline__ void load_cross(const LocalStage& a, int i, int j, int k, long long p, real_t f[3][7],
+                                           signed char m[3][6]) {
+  // addresses from the centre's pointers and uniform strides (x neighbours
+  // become immediate offsets of one address)
+  const real_t* c = a.us + p;
+  const Box& b = a.fbm.b;
+  const signed char* mc = (const signed char*)a.fbm.p + ((long long)(i - b.ilo) + (long long)a.fbm.sy * (j - b.jlo) +
+                                                         (long long)a.fbm.sz * (k - b.klo));
+  const long long st[3] = {1, a.sy, a.sz}, mst[3] = {1, a.fbm.sy, a.fbm.sz};
 #pragma unroll
   for (int d = 0; d < 3; d++) {
 #pragma unroll
-    for (int s = -3; s <= 3; s++) f[d][s + 3] = a.us[p + s * st[d]];
+    for (int s = -3; s <= 3; s++) f[d][s + 3] = c[s * st[d]];
 #pragma unroll
-    for (int s = -2; s <= 3; s++)
-      m[d][s + 2] = a.fbm.at(i + (d == 0 ? s : 0), j + (d == 1 ? s : 0), k + (d == 2 ? s : 0));
+    for (int s = -2; s <= 3; s++) m[d][s + 2] = mc[s * mst[d]];
   }
+}
+// Holds the loaded cross here: the compiler would otherwise sink the loads of
+// phi below the tests of the marks (and those below the point's own), one
+// dependent memory round trip each.  The empty asm takes every value, so all
+// loads are issued (and waited for together) before any is tested.
+__device__ __forceinline__ void hold(const real_t f[3][7], const signed char m[3][6]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int s = 0; s < 7; s++) asm volatile("" ::"v"(f[d][s]));
+#pragma unroll
+    for (int s = 0; s < 6; s++) asm volatile("" ::"v"((int)m[d][s]));
+  }
+}
+__device__ __forceinline__ bool cross_in_band(const signed char m[3][6]) {
   bool out = false;
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
     for (int s = 0; s < 6; s++) out |= m[d][s] == 0;
-  if (out) *flag = 1;
   return !out;
 }
-
 // Plus and minus WENO5 derivatives along axis d from the gathered cross, with
 // the differences HJ_WENO5_LOCAL stores (the 'big' value where a mark exceeds
 // mark_D1).
@@ -93,36 +111,9 @@ __device__ __forceinline__ void plus_minus_local(const LocalStage& a, const real
   }
 }
 
-// Plus/minus derivatives on all three axes at one point; false when the
-// stencil may not be evaluated there (flagged).
-__device__ __forceinline__ bool derivatives_local(const LocalStage& a, int i, int j, int k, long long p, real_t g[6],
-                                                  int* flag) {
-  real_t f[3][7];
-  signed char m[3][6];
-  if (!gather_cross(a, i, j, k, p, f, m, flag)) return false;
-  plus_minus_local(a, f[0], m[0], 0, g[0], g[3]);
-  plus_minus_local(a, f[1], m[1], 1, g[1], g[4]);
-  plus_minus_local(a, f[2], m[2], 2, g[2], g[5]);
-  return true;
-}
-
 __device__ __forceinline__ real_t local_sqrt(real_t x) {
   if (sizeof(real_t) == 8) return sqrt(x);
   return (real_t)__builtin_amdgcn_sqrtf((float)x);
-}
-
-// Level-0 entry t of the list: its point, offset in gb, and whether it takes
-// part (inside the band's box, mark <= mark_fb).
-__device__ __forceinline__ bool band_point(const LocalStage& a, long long t, int& i, int& j, int& k, long long& p,
-                                           int* flag) {
-  a.L.at(a.L.lo + (int)t, i, j, k);
-  if (!a.fbm.b.has(i, j, k) || !a.gb.has(i, j, k)) {
-    *flag = 1;
-    return false;
-  }
-  if (a.fbm.at(i, j, k) > a.fbm.mark) return false;
-  p = (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
-  return true;
 }
 
 // The RHS of one stage at a level-0 point: the Godunov normal-speed term
@@ -185,16 +176,43 @@ __global__ void __launch_bounds__(kT) local_normal_stage_kernel(LocalStage a, in
     return;
   }
   int i, j, k;
-  long long p;
-  if (!band_point(a, t, i, j, k, p, flag)) {
-    if (a.fbm.b.has(i, j, k)) pass_through<STAGE>(a, t, flag);  // mark above mark_fb
+  a.L.at(a.L.lo + (int)t, i, j, k);
+  if (!a.fbm.b.has(i, j, k) || !a.gb.has(i, j, k)) {
+    *flag = 1;
     return;
   }
+  const long long p = (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
+  if (!cross_in_boxes(a, i, j, k)) {  // an error unless the stage leaves the point
+    if (a.fbm.at(i, j, k) > a.fbm.mark)
+      pass_through<STAGE>(a, t, flag);
+    else
+      *flag = 1;
+    return;
+  }
+  // The point's own mark, its speed and phi_n, and the cross go out together
+  // (one memory round trip after the index list, not two); the cross of a
+  // point the stage does not update is loaded but never tested.
+  const signed char mark = a.fbm.at(i, j, k);
   const real_t v = a.vnf ? a.vnf[p] : a.vn;
   const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
+  real_t f[3][7];
+  signed char m[3][6];
+  load_cross(a, i, j, k, p, f, m);
+  hold(f, m);
+  asm volatile("" ::"v"((int)mark));
+  if (mark > a.fbm.mark) {  // mark above mark_fb
+    pass_through<STAGE>(a, t, flag);
+    return;
+  }
+  if (!cross_in_band(m)) {
+    *flag = 1;
+    return;
+  }
   real_t g[6];  // xp, yp, zp, xm, ym, zm
-  if (!derivatives_local(a, i, j, k, p, g, flag)) return;
-  const real_t us = a.us[p];
+  plus_minus_local(a, f[0], m[0], 0, g[0], g[3]);
+  plus_minus_local(a, f[1], m[1], 1, g[1], g[4]);
+  plus_minus_local(a, f[2], m[2], 2, g[2], g[5]);
+  const real_t us = f[0][3];
   a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, stage_rhs(a, v, g, us));
 }
 
@@ -208,21 +226,29 @@ __global__ void __launch_bounds__(kT) local_stage_from_rhs_kernel(LocalStage a, 
     pass_through<STAGE>(a, t, flag);
     return;
   }
+  const rhs_bits_t rb = reinterpret_cast<const rhs_bits_t*>(rhs)[t];  // with the index
   int i, j, k;
-  long long p;
-  if (!band_point(a, t, i, j, k, p, flag)) {
-    if (a.fbm.b.has(i, j, k)) pass_through<STAGE>(a, t, flag);
+  a.L.at(a.L.lo + (int)t, i, j, k);
+  if (!a.fbm.b.has(i, j, k) || !a.gb.has(i, j, k)) {
+    *flag = 1;
     return;
   }
-  const rhs_bits_t rb = reinterpret_cast<const rhs_bits_t*>(rhs)[t];
+  const long long p = (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
+  // the mark and the point's values together, before either is tested
+  const signed char mark = a.fbm.at(i, j, k);
+  const real_t us = a.us[p];
+  const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
+  asm volatile("" ::"v"((int)mark), "v"(us), "v"(ucv));
+  if (mark > a.fbm.mark) {  // mark above mark_fb
+    pass_through<STAGE>(a, t, flag);
+    return;
+  }
   if (rb == kSkipBits) {
     *flag = 1;
     return;
   }
   real_t r;
   __builtin_memcpy(&r, &rb, sizeof(r));
-  const real_t us = a.us[p];
-  const real_t ucv = STAGE == 1 ? real_t(0) : a.uc[p];
   a.out[p] = rk_combine<STAGE>(STAGE == 1 ? us : ucv, us, a.dt, r);
 }
 
@@ -236,20 +262,36 @@ struct LocalDtOp {
   real_t mdx2;
   int* flag;
   real_t* rhs;
-  __device__ bool operator()(long long t, int i, int j, int k, double& r) const {
-    if (!a.gb.has(i, j, k)) {
+  // The point's mark (M is a.fbm), its speed and the cross are loaded
+  // together; a point above mark_fb gives no value and no error.
+  __device__ bool marked(long long t, int i, int j, int k, const Mask& M, double& r) const {
+    if (!a.gb.has(i, j, k) || !cross_in_boxes(a, i, j, k)) {
+      if (M.at(i, j, k) > M.mark) return true;
       if (rhs) reinterpret_cast<rhs_bits_t*>(rhs)[t] = kSkipBits;
+      if (a.gb.has(i, j, k)) *flag = 1;  // the cross leaves gb or the band's box
       return false;
     }
     const long long p =
         (long long)(i - a.gb.ilo) + (long long)a.sy * (j - a.gb.jlo) + (long long)a.sz * (k - a.gb.klo);
-    real_t g[6];
-    if (!derivatives_local(a, i, j, k, p, g, flag)) {
+    const signed char mark = M.at(i, j, k);
+    const real_t v = a.vnf ? a.vnf[p] : a.vn;
+    real_t f[3][7];
+    signed char m[3][6];
+    load_cross(a, i, j, k, p, f, m);
+    hold(f, m);
+    asm volatile("" ::"v"((int)mark), "v"(v));
+    if (mark > M.mark) return true;
+    if (!cross_in_band(m)) {
+      *flag = 1;
       if (rhs) reinterpret_cast<rhs_bits_t*>(rhs)[t] = kSkipBits;
       return false;
     }
-    if (rhs) rhs[t] = stage_rhs(a, a.vnf ? a.vnf[p] : a.vn, g, a.us[p]);
-    const real_t av = a.vnf ? fabs(a.vnf[p]) : fabs(a.vn);
+    real_t g[6];  // xp, yp, zp, xm, ym, zm
+    plus_minus_local(a, f[0], m[0], 0, g[0], g[3]);
+    plus_minus_local(a, f[1], m[1], 1, g[1], g[4]);
+    plus_minus_local(a, f[2], m[2], 2, g[2], g[5]);
+    if (rhs) rhs[t] = stage_rhs(a, v, g, f[0][3]);
+    const real_t av = fabs(v);
     real_t n;
     const real_t h = stencil::normal_h_over_dx(av, g[0], g[1], g[2], g[3], g[4], g[5], mdx2, a.inv[0], a.inv[1],
                                                a.inv[2], &n);

@@ -10,6 +10,8 @@
 // reports kErrIndex) where the reference would read or write out of bounds.
 #pragma once
 
+#include <type_traits>
+
 #include <cstring>
 
 #include "lsmhip_core.hpp"
@@ -163,6 +165,13 @@ __device__ __forceinline__ bool call_max_op(const Op& op, long long, int i, int 
   return op(i, j, k, r);
 }
 
+// An op with marked(t, i, j, k, M, r) reads the point's mark itself, with its
+// own loads, and returns true without a value where the mark exceeds M.mark.
+template <class Op, class = void>
+struct reads_own_mark : std::false_type {};
+template <class Op>
+struct reads_own_mark<Op, std::void_t<decltype(&Op::marked)>> : std::true_type {};
+
 // Max over admissible points of non-negative values (-1 stands for "no
 // value": a point that is not admissible, or an op that declines it).
 template <class Op>
@@ -174,8 +183,13 @@ __global__ void __launch_bounds__(kT) points_max_kernel(IndexList L, Mask M, Op 
   if (t <= (long long)L.hi - L.lo) {
     int i, j, k;
     L.at(L.lo + (int)t, i, j, k);
-    if (!M.b.has(i, j, k)) *flag = 1;
-    else if (M.at(i, j, k) <= M.mark && !call_max_op(op, t, i, j, k, m, 0)) *flag = 1;
+    if (!M.b.has(i, j, k)) {
+      *flag = 1;
+    } else if constexpr (reads_own_mark<Op>::value) {
+      if (!op.marked(t, i, j, k, M, m)) *flag = 1;
+    } else {
+      if (M.at(i, j, k) <= M.mark && !call_max_op(op, t, i, j, k, m, 0)) *flag = 1;
+    }
   }
   m = reduce::wave_max(m);
   __shared__ double w[kT / 64];
