@@ -171,6 +171,7 @@ NEIGHBOUR_KERNELS = [
     ("lsm3dcomputegaussiancurvatureorder4", 5, 1),
     ("lsm3dcomputeorthogonalizationeqnrhs", 13, 1),
     ("lsm3dcomputefieldextensioneqnrhs", 7, 1),
+    ("lsm3dcentralhessian", 1, 6),
 ]
 
 
@@ -223,11 +224,11 @@ def kernel_table(n, reps, warmup):
         A["S"] = torch.sin(4 * A["phi"]) + A["px"]
         A["w"] = A["S"] * A["pz"]
         del r
-        out = [torch.zeros_like(A["phi"]) for _ in range(3)]
+        out = [torch.zeros_like(A["phi"]) for _ in range(6)]
         P = A
         calls = {
-            "lsm3dcomputeunitnormal": (*out, gb, P["px"], P["py"], P["pz"], gb, fb),
-            "lsm3dcomputesignedunitnormal": (*out, gb, P["px"], P["py"], P["pz"], gb, P["phi"], gb, fb, h, h, h),
+            "lsm3dcomputeunitnormal": (*out[:3], gb, P["px"], P["py"], P["pz"], gb, fb),
+            "lsm3dcomputesignedunitnormal": (*out[:3], gb, P["px"], P["py"], P["pz"], gb, P["phi"], gb, fb, h, h, h),
             "lsm3dcomputemeancurvature": (out[0], gb, P["px"], P["py"], P["pz"], gb, P["pxx"], P["pxy"], P["pxz"],
                                           P["pyy"], P["pyz"], P["pzz"], gb, fb),
             "lsm3dcomputegaussiancurvatureorder2": (out[0], gb, P["phi"], gb, P["px"], P["py"], P["pz"], P["gmag"],
@@ -242,6 +243,7 @@ def kernel_table(n, reps, warmup):
                                                     P["S"], P["gmag"], P["w"], gb, fb, h, h, h),
             "lsm3dcomputefieldextensioneqnrhs": (out[0], gb, P["S"], gb, P["phi"], gb, P["pxx"], P["pyy"], P["pzz"],
                                                  gb, P["px"], P["py"], P["pz"], gb, fb, h, h, h),
+            "lsm3dcentralhessian": (*out, gb, P["phi"], gb, fb, h, h, h),
         }
         cells = n ** 3
         size = 8 if prec == "f64" else 4

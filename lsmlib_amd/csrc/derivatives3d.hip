@@ -103,29 +103,26 @@ struct Central4Line {
 
 // lsm3dCentralHessian (src/toolbox/lsm_spatial_derivatives3d.f.in:3658-3715):
 // compact second differences over dx*dx etc., mixed ones as 0.25 (f(+,+) -
-// f(-,+) - f(+,-) + f(-,-)) / (dx*dy), in that operation order; the 19 values
-// of the cell loaded before any store.
+// f(-,+) - f(+,-) + f(-,-)) / (dx*dy), in that operation order, on the plane
+// march over the 19-point cross.
 struct Hessian6 {
   View<real_t> xx, xy, xz, yy, yz, zz;
 };
-__global__ void __launch_bounds__(256) central_hessian_kernel(Box fb, Hessian6 out, View<const real_t> phi,
-                                                              real_t dxx, real_t dxy, real_t dxz, real_t dyy,
-                                                              real_t dyz, real_t dzz) {
-  LSMHIP_CELL(fb);
-  const real_t c = phi(i, j, k);
-  const real_t xp = phi(i + 1, j, k), xm = phi(i - 1, j, k), yp = phi(i, j + 1, k), ym = phi(i, j - 1, k),
-               zp = phi(i, j, k + 1), zm = phi(i, j, k - 1);
-  const real_t xy[4] = {phi(i + 1, j + 1, k), phi(i - 1, j + 1, k), phi(i + 1, j - 1, k), phi(i - 1, j - 1, k)};
-  const real_t xz[4] = {phi(i + 1, j, k + 1), phi(i - 1, j, k + 1), phi(i + 1, j, k - 1), phi(i - 1, j, k - 1)};
-  const real_t yz[4] = {phi(i, j + 1, k + 1), phi(i, j - 1, k + 1), phi(i, j + 1, k - 1), phi(i, j - 1, k - 1)};
-  const real_t q = real_t(0.25);
-  out.xx(i, j, k) = (xp - real_t(2) * c + xm) / dxx;
-  out.xy(i, j, k) = q * (xy[0] - xy[1] - xy[2] + xy[3]) / dxy;
-  out.xz(i, j, k) = q * (xz[0] - xz[1] - xz[2] + xz[3]) / dxz;
-  out.yy(i, j, k) = (yp - real_t(2) * c + ym) / dyy;
-  out.yz(i, j, k) = q * (yz[0] - yz[1] - yz[2] + yz[3]) / dyz;
-  out.zz(i, j, k) = (zp - real_t(2) * c + zm) / dzz;
-}
+struct HessianMarch {
+  Hessian6 out;
+  real_t dxx, dxy, dxz, dyy, dyz, dzz;
+  struct In {};
+  __device__ In load(int, int, int) const { return In{}; }
+  __device__ void store(int i, int j, int k, const march::Cross19& c, const In&) const {
+    const real_t q = real_t(0.25);
+    out.xx(i, j, k) = (c.xp - real_t(2) * c.c + c.xm) / dxx;
+    out.xy(i, j, k) = q * (c.xy[0] - c.xy[2] - c.xy[1] + c.xy[3]) / dxy;
+    out.xz(i, j, k) = q * (c.xz[0] - c.xz[2] - c.xz[1] + c.xz[3]) / dxz;
+    out.yy(i, j, k) = (c.yp - real_t(2) * c.c + c.ym) / dyy;
+    out.yz(i, j, k) = q * (c.yz[0] - c.yz[2] - c.yz[1] + c.yz[3]) / dyz;
+    out.zz(i, j, k) = (c.zp - real_t(2) * c.c + c.zm) / dzz;
+  }
+};
 
 // One axis of lsm3dPhiUpwindGradF (src/toolbox/lsm_spatial_derivatives3d.f.in:
 // 3270-3458): the one-sided derivative from the neighbour whose phi is nearer
@@ -335,10 +332,12 @@ LSMHIP_EXPORT void lsm3dcentralhessian_(real_t* pxx, real_t* pxy, real_t* pxz, r
   if (!check_box(hg, fb, who, "hessian") || !check_box(pg, fb.widen(1, 1, 1), who, "phi") || !check_size(hg, who) ||
       !check_size(pg, who))
     return;
-  hipLaunchKernelGGL(central_hessian_kernel, grid_for(fb), block2d(), 0, stream(), fb,
-                     Hessian6{view(pxx, hg), view(pxy, hg), view(pxz, hg), view(pyy, hg), view(pyz, hg),
-                              view(pzz, hg)},
-                     view(phi, pg), *dx * *dx, *dx * *dy, *dx * *dz, *dy * *dy, *dy * *dz, *dz * *dz);
+  const march::Tiles tl = march::tiles_for(fb);
+  hipLaunchKernelGGL(march::cross19_march_kernel<HessianMarch>, march::grid_for(tl), march::block(), 0, stream(), fb,
+                     view(phi, pg), tl,
+                     HessianMarch{Hessian6{view(pxx, hg), view(pxy, hg), view(pxz, hg), view(pyy, hg), view(pyz, hg),
+                                           view(pzz, hg)},
+                                  *dx * *dx, *dx * *dy, *dx * *dz, *dy * *dy, *dy * *dz, *dz * *dz});
   finish(who);
 }
 

@@ -16,6 +16,7 @@
 // contraction, and every result is bitwise the reference's except the mean
 // curvature's |grad phi|^3 = (|grad phi|^2)^1.5 (see stencil::pow_three_halves).
 #include "lsmhip_core.hpp"
+#include "march.hpp"
 #include "stencil.hpp"
 
 using namespace lsmhip;
@@ -253,47 +254,31 @@ struct Factors2 {
   real_t xx, yy, zz, xy, xz, yz;
 };
 
-// phi at a cell and its four in-plane face neighbours.
-struct Cross {
-  real_t c, xm, xp, ym, yp;
-};
-__device__ __forceinline__ Cross cross_at(const real_t* __restrict__ F, int o, int sy) {
-  return Cross{F[o], F[o - 1], F[o + 1], F[o - sy], F[o + sy]};
-}
-
 // lsm_curvature3d.f.in:136-248: second differences of phi (2-point cross
-// stencil for the mixed terms), numerator over |grad phi|^4.  Window: the
-// cross of phi on planes k-1, k, k+1.
-template <int KC>
-__global__ void __launch_bounds__(256)
-    gaussian_order2_kernel(Tiles T, real_t* __restrict__ kappa, Lay lk, const real_t* __restrict__ F, Lay lf,
-                           const real_t* __restrict__ gx, const real_t* __restrict__ gy,
-                           const real_t* __restrict__ gz, const real_t* __restrict__ gmag, Lay lg, Factors2 f) {
-  Column c;
-  if (!column(T, c)) return;
-  const int sy = lf.sy, sz = lf.sz;
-  int of = lf.at(c.i, c.j, c.k0), og = lg.at(c.i, c.j, c.k0), ok = lk.at(c.i, c.j, c.k0);
-  Cross w0 = cross_at(F, of - sz, sy), w1 = cross_at(F, of, sy), w2 = cross_at(F, of + sz, sy);
-  march<KC>(c.n, [&](int t) {
-    const int o = of + t * sz, q = og + t * lg.sz;
-    const real_t xpyp = F[o + 1 + sy], xpym = F[o + 1 - sy], xmyp = F[o - 1 + sy], xmym = F[o - 1 - sy];
-    const real_t x = gx[q], y = gy[q], z = gz[q], gm = gmag[q];
-    const Cross nx = t + 1 < c.n ? cross_at(F, o + 2 * sz, sy) : w2;
-    const real_t xx = (w1.xp - real_t(2) * w1.c + w1.xm) * f.xx;
-    const real_t yy = (w1.yp - real_t(2) * w1.c + w1.ym) * f.yy;
-    const real_t zz = (w2.c - real_t(2) * w1.c + w0.c) * f.zz;
-    const real_t xy = (xpyp - xpym - xmyp + xmym) * f.xy;
-    const real_t xz = (w2.xp - w0.xp - w2.xm + w0.xm) * f.xz;
-    const real_t yz = (w2.yp - w0.yp - w2.ym + w0.ym) * f.yz;
-    const real_t den = gm * gm * gm * gm;
+// stencil for the mixed terms), numerator over |grad phi|^4, on the plane
+// march over the 19-point cross; the gradient and its magnitude loaded per
+// cell.
+struct GaussCurvMarch {
+  View<real_t> kappa;
+  View<const real_t> px, py, pz, gmag;
+  Factors2 f;
+  struct In {
+    real_t x, y, z, gm;
+  };
+  __device__ In load(int i, int j, int k) const { return In{px(i, j, k), py(i, j, k), pz(i, j, k), gmag(i, j, k)}; }
+  __device__ void store(int i, int j, int k, const lsmhip::march::Cross19& c, const In& v) const {
+    const real_t xx = (c.xp - real_t(2) * c.c + c.xm) * f.xx;
+    const real_t yy = (c.yp - real_t(2) * c.c + c.ym) * f.yy;
+    const real_t zz = (c.zp - real_t(2) * c.c + c.zm) * f.zz;
+    const real_t xy = (c.xy[0] - c.xy[1] - c.xy[2] + c.xy[3]) * f.xy;
+    const real_t xz = (c.xz[0] - c.xz[1] - c.xz[2] + c.xz[3]) * f.xz;
+    const real_t yz = (c.yz[0] - c.yz[1] - c.yz[2] + c.yz[3]) * f.yz;
+    const real_t den = v.gm * v.gm * v.gm * v.gm;
     real_t out = real_t(0);
-    if (!(den < kZeroTol)) out = gauss_num(x, y, z, xx, yy, zz, xy, xz, yz) / den;
-    kappa[ok + t * lk.sz] = out;
-    w0 = w1;
-    w1 = w2;
-    w2 = nx;
-  });
-}
+    if (!(den < kZeroTol)) out = gauss_num(v.x, v.y, v.z, xx, yy, zz, xy, xz, yz) / den;
+    kappa(i, j, k) = out;
+  }
+};
 
 struct Factors4 {
   real_t xx, yy, zz;  // 1/(12 h^2)
@@ -494,9 +479,10 @@ LSMHIP_EXPORT void lsm3dcomputegaussiancurvatureorder2_(real_t* kappa, LSMHIP_BO
     return;
   Factors2 f{real_t(1.0) / *dx / *dx, real_t(1.0) / *dy / *dy, real_t(1.0) / *dz / *dz,
              real_t(0.25) / *dx / *dy, real_t(0.25) / *dx / *dz, real_t(0.25) / *dy / *dz};
-  const Tiles t = tiles(fb, kMarchKC);
-  hipLaunchKernelGGL(gaussian_order2_kernel<kMarchKC>, tile_grid(t), block2d(), 0, stream(), t, kappa, lay(kg), phi,
-                     lay(pg), px, py, pz, gmag, lay(gg), f);
+  const lsmhip::march::Tiles tl = lsmhip::march::tiles_for(fb);
+  hipLaunchKernelGGL(lsmhip::march::cross19_march_kernel<GaussCurvMarch>, lsmhip::march::grid_for(tl),
+                     lsmhip::march::block(), 0, stream(), fb, view(phi, pg), tl,
+                     GaussCurvMarch{view(kappa, kg), view(px, gg), view(py, gg), view(pz, gg), view(gmag, gg), f});
   finish(who);
 }
 

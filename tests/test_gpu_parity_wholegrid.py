@@ -133,7 +133,7 @@ def test_whole_grid_bitwise_odd_plane_count(hip, sym):
 
 
 PLANE_MARCH = ["lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlaplacianorder2_",
-               "lsm3dcomputemeancurvatureorder2_",
+               "lsm3dcomputemeancurvatureorder2_", "lsm3dcomputegaussiancurvatureorder2_", "lsm3dcentralhessian_",
                "lsm3dhjeno1_", "lsm3dhjeno2_", "lsm3dhjeno3_", "lsm3dhjweno5_", "lsm3dupwindhjeno1_",
                "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_", "lsm3dupwindhjweno5_"]
 
