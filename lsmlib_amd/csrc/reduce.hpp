@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstring>
+#include <type_traits>
 
 #include "lsmhip_core.hpp"
 
@@ -52,12 +53,28 @@ __device__ __forceinline__ Pair wave_sum(Pair v) {
 
 // Cells are visited one row (fixed j, k) per wave at a time: the lanes stride
 // along i, so loads are coalesced and the row index is split into (j, k) once
-// per row rather than once per cell; each lane issues kUnroll loads before it
+// per row rather than once per cell; each lane issues kLoads loads before it
 // accumulates, into kUnroll independent accumulators (no serial dependence
 // between them), which the caller combines in a fixed order.  Out-of-row
 // slots contribute the identity (0 for the maxima, whose terms are >= 0; 0 for
 // the sums, which leaves a Neumaier pair as is).
 constexpr int kUnroll = 4;
+// Slots loaded per lane and pass: twice kUnroll for 4-byte reals, so that a
+// wave has as many bytes in flight as with doubles (slot u accumulates into
+// acc[u % kUnroll]).
+constexpr int kLoads = sizeof(real_t) == 4 ? 2 * kUnroll : kUnroll;
+
+//
+// An op with `In load(i, j, k)` and `double eval(const In&, i, j, k)` has its
+// unconditional loads split from its arithmetic: the kLoads loads of a lane
+// go out together (at clamped, in-box indices), and only then are the terms
+// evaluated.  Called whole, an op's branches would keep the compiler from
+// issuing the next slot's loads before this slot's term, one memory round
+// trip per slot.
+template <class Op, class = void>
+struct split_op : std::false_type {};
+template <class Op>
+struct split_op<Op, std::void_t<typename Op::In>> : std::true_type {};
 
 template <class Op, class Acc, class F>
 __device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[kUnroll], F combine,
@@ -68,15 +85,26 @@ __device__ __forceinline__ void for_rows(const Box& b, const Op& op, Acc (&acc)[
   const long long nwaves = (long long)gridDim.x * (kThreads / 64);
   for (long long r = blockIdx.x * (long long)(kThreads / 64) + (threadIdx.x >> 6); r < rows; r += nwaves) {
     const int j = b.jlo + (int)(r % ny), k = b.klo + (int)(r / ny);
-    for (int i0 = lane; i0 < nx; i0 += 64 * kUnroll) {
-      double v[kUnroll];
+    for (int i0 = lane; i0 < nx; i0 += 64 * kLoads) {
+      double v[kLoads];
+      if constexpr (split_op<Op>::value) {
+        typename Op::In in[kLoads];
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) {
-        const int i = i0 + 64 * u;
-        v[u] = i < nx ? op(b.ilo + i, j, k) : identity;
+        for (int u = 0; u < kLoads; u++) in[u] = op.load(b.ilo + min(i0 + 64 * u, nx - 1), j, k);
+#pragma unroll
+        for (int u = 0; u < kLoads; u++) {
+          const int i = i0 + 64 * u;
+          v[u] = i < nx ? op.eval(in[u], b.ilo + i, j, k) : identity;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < kLoads; u++) {
+          const int i = i0 + 64 * u;
+          v[u] = i < nx ? op(b.ilo + i, j, k) : identity;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++) acc[u] = combine(acc[u], v[u]);
+      for (int u = 0; u < kLoads; u++) acc[u % kUnroll] = combine(acc[u % kUnroll], v[u]);
     }
   }
 }

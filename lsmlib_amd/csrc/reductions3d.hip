@@ -22,27 +22,45 @@ constexpr double kPi = 3.14159265358979323846;
 // Truncated 1/pi of the reference's Fortran Heaviside (lsm_geometry3d_fort.f.in:432-433).
 constexpr double kOneOverPi = 0.31830988618379;
 
+// Each op: operator()(i, j, k) = eval(load(i, j, k), i, j, k); load reads what
+// every cell needs, eval may read more where the term needs it.
 struct MaxNormDiff {
   View<const real_t> a, b;
-  __device__ double operator()(int i, int j, int k) const { return (double)fabs(a(i, j, k) - b(i, j, k)); }
+  struct In {
+    real_t a, b;
+  };
+  __device__ In load(int i, int j, int k) const { return In{a(i, j, k), b(i, j, k)}; }
+  __device__ double eval(const In& v, int, int, int) const { return (double)fabs(v.a - v.b); }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 struct AdvectionCfl {
   View<const real_t> u, v, w;
   real_t ix, iy, iz;
-  __device__ double operator()(int i, int j, int k) const {
-    return (double)(fabs(u(i, j, k)) * ix + fabs(v(i, j, k)) * iy + fabs(w(i, j, k)) * iz);
+  struct In {
+    real_t u, v, w;
+  };
+  __device__ In load(int i, int j, int k) const { return In{u(i, j, k), v(i, j, k), w(i, j, k)}; }
+  __device__ double eval(const In& a, int, int, int) const {
+    return (double)(fabs(a.u) * ix + fabs(a.v) * iy + fabs(a.w) * iz);
   }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 struct NormalCfl {
   View<const real_t> vn, xp, yp, zp, xm, ym, zm;
   real_t vconst, mdx2, ix, iy, iz;
-  __device__ double operator()(int i, int j, int k) const {
-    const real_t av = vn.p ? fabs(vn(i, j, k)) : vconst;
-    return (double)stencil::normal_h_over_dx(av, xp(i, j, k), yp(i, j, k), zp(i, j, k), xm(i, j, k), ym(i, j, k),
-                                             zm(i, j, k), mdx2, ix, iy, iz);
+  struct In {
+    real_t av, xp, yp, zp, xm, ym, zm;
+  };
+  __device__ In load(int i, int j, int k) const {
+    return In{vn.p ? fabs(vn(i, j, k)) : vconst, xp(i, j, k), yp(i, j, k), zp(i, j, k), xm(i, j, k), ym(i, j, k),
+              zm(i, j, k)};
   }
+  __device__ double eval(const In& a, int, int, int) const {
+    return (double)stencil::normal_h_over_dx(a.av, a.xp, a.yp, a.zp, a.xm, a.ym, a.zm, mdx2, ix, iy, iz);
+  }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 // sign = -1: smoothed volume of {phi < 0}; sign = +1: of {phi > 0}.  F.p null means F = 1.
@@ -50,10 +68,13 @@ struct VolumeTerm {
   View<const real_t> phi, F;
   real_t eps, dV;
   int sign;
-  __device__ double operator()(int i, int j, int k) const {
+  struct In {
+    real_t p, f;
+  };
+  __device__ In load(int i, int j, int k) const { return In{phi(i, j, k), F.p ? F(i, j, k) : real_t(1)}; }
+  __device__ double eval(const In& v, int, int, int) const {
     // q = phi / eps only inside the smoothing band, where it is used
-    const real_t p = phi(i, j, k);
-    const real_t f = F.p ? F(i, j, k) : real_t(1);
+    const real_t p = v.p, f = v.f;
     if (sign < 0) {
       if (p < -eps) return (double)(F.p ? f * dV : dV);
       if (p < eps) {
@@ -71,19 +92,26 @@ struct VolumeTerm {
     }
     return 0.0;
   }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 struct SurfaceTerm {
   View<const real_t> phi, F, x, y, z;
   real_t eps, ie, dV;
-  __device__ double operator()(int i, int j, int k) const {
-    const real_t p = phi(i, j, k);
+  struct In {
+    real_t p;
+  };
+  __device__ In load(int i, int j, int k) const { return In{phi(i, j, k)}; }
+  // the gradient (and F) only in the smoothing band
+  __device__ double eval(const In& v, int i, int j, int k) const {
+    const real_t p = v.p;
     if (!(fabs(p) < eps)) return 0.0;
     const real_t delta = real_t(0.5) * ie * (real_t(1) + cos(real_t(kPi) * p * ie));
     const real_t gx = x(i, j, k), gy = y(i, j, k), gz = z(i, j, k);
     const real_t n = sqrt(gx * gx + gy * gy + gz * gz);
     return (double)(F.p ? delta * n * F(i, j, k) * dV : delta * n * dV);
   }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 // The cells of a control volume (src/toolbox/lsm_math_utils3d.f.in:852-1844):
@@ -128,8 +156,12 @@ struct MaxNormDiffCv {
 struct SignCount {
   View<const real_t> phi, cv;
   int sign, sgn;
-  __device__ double operator()(int i, int j, int k) const {
-    const real_t p = phi(i, j, k);
+  struct In {
+    real_t p;
+  };
+  __device__ In load(int i, int j, int k) const { return In{phi(i, j, k)}; }
+  __device__ double eval(const In& v, int i, int j, int k) const {
+    const real_t p = v.p;
     if (!(sign > 0 ? p > real_t(0) : p < real_t(0))) return 0.0;
     if (cv.p) {
       const real_t c = cv(i, j, k);
@@ -137,18 +169,25 @@ struct SignCount {
     }
     return 1.0;
   }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 // delta_phi |grad phi| dV where a given delta function is positive
 // (lsm_geometry3d_fort.f.in:669-745).
 struct DeltaAreaTerm {
   View<const real_t> d, x, y, z;
   real_t dV;
-  __device__ double operator()(int i, int j, int k) const {
-    const real_t dp = d(i, j, k);
+  struct In {
+    real_t dp;
+  };
+  __device__ In load(int i, int j, int k) const { return In{d(i, j, k)}; }
+  // the gradient only where the delta function is positive
+  __device__ double eval(const In& v, int i, int j, int k) const {
+    const real_t dp = v.dp;
     if (!(dp > real_t(0))) return 0.0;
     const real_t gx = x(i, j, k), gy = y(i, j, k), gz = z(i, j, k);
     return (double)(dp * sqrt(gx * gx + gy * gy + gz * gz) * dV);
   }
+  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 bool ok(const char* who, const Box& ib, std::initializer_list<std::pair<Box, const char*>> arrays) {
