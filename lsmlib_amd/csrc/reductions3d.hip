@@ -22,29 +22,21 @@ constexpr double kPi = 3.14159265358979323846;
 // Truncated 1/pi of the reference's Fortran Heaviside (lsm_geometry3d_fort.f.in:432-433).
 constexpr double kOneOverPi = 0.31830988618379;
 
-// Each op: operator()(i, j, k) = eval(load(i, j, k), i, j, k); load reads what
-// every cell needs, eval may read more where the term needs it.
+// Ops with branches on their loaded values are split: operator()(i, j, k) =
+// eval(load(i, j, k), i, j, k), where load reads what every cell needs and
+// eval may read more where the term needs it.  The two branch-free maxima
+// below are faster called whole.
 struct MaxNormDiff {
   View<const real_t> a, b;
-  struct In {
-    real_t a, b;
-  };
-  __device__ In load(int i, int j, int k) const { return In{a(i, j, k), b(i, j, k)}; }
-  __device__ double eval(const In& v, int, int, int) const { return (double)fabs(v.a - v.b); }
-  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
+  __device__ double operator()(int i, int j, int k) const { return (double)fabs(a(i, j, k) - b(i, j, k)); }
 };
 
 struct AdvectionCfl {
   View<const real_t> u, v, w;
   real_t ix, iy, iz;
-  struct In {
-    real_t u, v, w;
-  };
-  __device__ In load(int i, int j, int k) const { return In{u(i, j, k), v(i, j, k), w(i, j, k)}; }
-  __device__ double eval(const In& a, int, int, int) const {
-    return (double)(fabs(a.u) * ix + fabs(a.v) * iy + fabs(a.w) * iz);
+  __device__ double operator()(int i, int j, int k) const {
+    return (double)(fabs(u(i, j, k)) * ix + fabs(v(i, j, k)) * iy + fabs(w(i, j, k)) * iz);
   }
-  __device__ double operator()(int i, int j, int k) const { return eval(load(i, j, k), i, j, k); }
 };
 
 struct NormalCfl {
