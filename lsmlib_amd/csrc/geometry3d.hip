@@ -320,13 +320,12 @@ __global__ void __launch_bounds__(256)
     const real_t yz = (-y4 + real_t(8) * y3 + y0 - real_t(8) * y1) * f.z;
     const real_t zx = (-zxp2 + real_t(8) * zxp1 + zxm2 - real_t(8) * zxm1) * f.x;
     const real_t den = GAUSS ? gm * gm * gm * gm : gm * gm * gm;
-    real_t out = real_t(0);
-    if (!(den < kZeroTol)) {
-      const real_t num = GAUSS ? gauss_num(x, y, z, xx, yy, zz, xy, zx, yz)
-                               : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, y, z);
-      out = num / den;
-    }
-    kappa[ok + t * lk.sz] = out;
+    // the quotient formed in every cell and then selected, so that no branch
+    // separates one cell's loads from the next one's
+    const real_t num = GAUSS ? gauss_num(x, y, z, xx, yy, zz, xy, zx, yz)
+                             : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, y, z);
+    const real_t quot = num / den;
+    kappa[ok + t * lk.sz] = den < kZeroTol ? real_t(0) : quot;
     p0 = p1; p1 = p2; p2 = p3; p3 = p4; p4 = p5;
     y0 = y1; y1 = y2; y2 = y3; y3 = y4; y4 = y5;
   });
