@@ -290,45 +290,118 @@ struct Factors4 {
 // derivatives (:342-351): phi_xy = d/dy phi_x, phi_yz = d/dz phi_y,
 // phi_zx = d/dx phi_z.  GAUSS = 0: mean curvature over |grad phi|^3 (numerator
 // order of :358-366); GAUSS = 1: Gaussian curvature over |grad phi|^4.
-// Windows: phi and phi_y at k-2 .. k+2.
-template <int KC, int GAUSS>
-__global__ void __launch_bounds__(256)
-    curvature_order4_kernel(Tiles T, real_t* __restrict__ kappa, Lay lk, const real_t* __restrict__ F, Lay lf,
-                            const real_t* __restrict__ gx, const real_t* __restrict__ gy,
-                            const real_t* __restrict__ gz, const real_t* __restrict__ gmag, Lay lg, Factors4 f) {
-  Column c;
-  if (!column(T, c)) return;
-  const int sy = lf.sy, sz = lf.sz, gsy = lg.sy, gsz = lg.sz;
-  const int of = lf.at(c.i, c.j, c.k0), og = lg.at(c.i, c.j, c.k0), ok = lk.at(c.i, c.j, c.k0);
-  real_t p0 = F[of - 2 * sz], p1 = F[of - sz], p2 = F[of], p3 = F[of + sz], p4 = F[of + 2 * sz];
-  real_t y0 = gy[og - 2 * gsz], y1 = gy[og - gsz], y2 = gy[og], y3 = gy[og + gsz], y4 = gy[og + 2 * gsz];
-  march<KC>(c.n, [&](int t) {
-    const int o = of + t * sz, q = og + t * gsz;
-    const real_t fxm2 = F[o - 2], fxm1 = F[o - 1], fxp1 = F[o + 1], fxp2 = F[o + 2];
-    const real_t fym2 = F[o - 2 * sy], fym1 = F[o - sy], fyp1 = F[o + sy], fyp2 = F[o + 2 * sy];
-    const real_t xym2 = gx[q - 2 * gsy], xym1 = gx[q - gsy], xyp1 = gx[q + gsy], xyp2 = gx[q + 2 * gsy];
-    const real_t zxm2 = gz[q - 2], zxm1 = gz[q - 1], zxp1 = gz[q + 1], zxp2 = gz[q + 2];
-    const real_t x = gx[q], z = gz[q], gm = gmag[q];
-    const bool more = t + 1 < c.n;
-    const real_t p5 = more ? F[o + 3 * sz] : p4;
-    const real_t y5 = more ? gy[q + 3 * gsz] : y4;
-    const real_t y = y2;
-    const real_t xx = (-fxp2 + real_t(16) * fxp1 - real_t(30) * p2 - fxm2 + real_t(16) * fxm1) * f.xx;
-    const real_t yy = (-fyp2 + real_t(16) * fyp1 - real_t(30) * p2 - fym2 + real_t(16) * fym1) * f.yy;
-    const real_t zz = (-p4 + real_t(16) * p3 - real_t(30) * p2 - p0 + real_t(16) * p1) * f.zz;
-    const real_t xy = (-xyp2 + real_t(8) * xyp1 + xym2 - real_t(8) * xym1) * f.y;
-    const real_t yz = (-y4 + real_t(8) * y3 + y0 - real_t(8) * y1) * f.z;
-    const real_t zx = (-zxp2 + real_t(8) * zxp1 + zxm2 - real_t(8) * zxm1) * f.x;
-    const real_t den = GAUSS ? gm * gm * gm * gm : gm * gm * gm;
-    // the quotient formed in every cell and then selected, so that no branch
-    // separates one cell's loads from the next one's
-    const real_t num = GAUSS ? gauss_num(x, y, z, xx, yy, zz, xy, zx, yz)
-                             : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, y, z);
-    const real_t quot = num / den;
-    kappa[ok + t * lk.sz] = den < kZeroTol ? real_t(0) : quot;
-    p0 = p1; p1 = p2; p2 = p3; p3 = p4; p4 = p5;
-    y0 = y1; y1 = y2; y2 = y3; y3 = y4; y4 = y5;
-  });
+// On the plane march (64 x 8 tiles, chunks of 16 planes): phi with its x and
+// y neighbours, phi_x with its y neighbours and phi_z with its x neighbours
+// staged per plane in LDS (two-cell halos), phi and phi_y along z carried in
+// registers, every next-plane value in flight while the plane is computed.
+// The quotient is formed in every cell and then selected.  g (phi_x, phi_y,
+// phi_z, |grad phi|) and phi must cover fb widened by two cells.
+template <int GAUSS>
+__global__ void __launch_bounds__(lsmhip::march::kMX* lsmhip::march::kMY)
+    curvature_order4_march_kernel(Box fb, View<const real_t> F, View<const real_t> gx, View<const real_t> gy,
+                                  View<const real_t> gz, View<const real_t> gmag, View<real_t> kappa,
+                                  lsmhip::march::Tiles tl, Factors4 f) {
+  constexpr int MX = lsmhip::march::kMX, MY = lsmhip::march::kMY, R = 2;
+  constexpr int PX = MX + 2 * R, PY = MY + 2 * R;
+  constexpr int NP = PX * PY - MX * MY, NX = 2 * R * MX, NZ = 2 * R * MY;
+  static_assert(NP <= MX * MY && NX <= MX * MY, "one halo element per thread at most");
+  __shared__ real_t P[PX * PY], X[MX * PY], Z[PX * MY];
+  const int tid = threadIdx.y * MX + threadIdx.x;
+  const int nt = tl.nti * tl.ntj * tl.ntk;
+  const int per = (nt + 7) / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= nt) return;
+  const int ti = t % tl.nti, tj = (t / tl.nti) % tl.ntj, tk = t / (tl.nti * tl.ntj);
+  const int i0 = fb.ilo + ti * MX, j0 = fb.jlo + tj * MY;
+  const int kb = fb.klo + tk * tl.kc;
+  const int ke = min(kb + tl.kc - 1, fb.khi);
+  const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+  const bool active = i <= fb.ihi && j <= fb.jhi;
+  // within two cells of fb: inside phi's and g's boxes; columns further out are never read
+  auto near = [&](int a, int b) { return a <= fb.ihi + R && b <= fb.jhi + R; };
+  const bool own_in = near(i, j);
+  const int psz = F.sz, gsz = gx.sz;
+  const int op = own_in ? F.offset(i, j, kb) : 0, og = own_in ? gx.offset(i, j, kb) : 0;
+  int hp = -1, hx = -1, hz = -1, sp = 0, sx = 0, szs = 0;
+  if (tid < NP) {
+    int px, py;
+    lsmhip::march::halo_pos<R>(tid, px, py);
+    sp = py * PX + px;
+    if (near(i0 - R + px, j0 - R + py)) hp = F.offset(i0 - R + px, j0 - R + py, kb);
+  }
+  if (tid < NX) {  // rows 0, 1 and MY + 2, MY + 3 of the phi_x plane
+    const int r = tid / MX, row = r < R ? r : r + MY, px = tid % MX;
+    sx = row * MX + px;
+    if (near(i0 + px, j0 - R + row)) hx = gx.offset(i0 + px, j0 - R + row, kb);
+  }
+  if (tid < NZ) {  // columns 0, 1 and MX + 2, MX + 3 of the phi_z plane
+    const int row = tid / (2 * R), c = tid % (2 * R), px = c < R ? c : MX + c;
+    szs = row * PX + px;
+    if (near(i0 - R + px, j0 + row)) hz = gz.offset(i0 - R + px, j0 + row, kb);
+  }
+  const int slp = (threadIdx.y + R) * PX + threadIdx.x + R, slx = (threadIdx.y + R) * MX + threadIdx.x,
+            slz = threadIdx.y * PX + threadIdx.x + R;
+  // p[s] = phi(i, j, k - 2 + s), y[s] = phi_y(i, j, k - 2 + s); [5] loading
+  real_t p[6], y[6];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    p[s] = own_in ? F.p[op + (s - R) * psz] : real_t(0);
+    y[s] = active ? gy.p[og + (s - R) * gsz] : real_t(0);
+  }
+  real_t cx = own_in ? gx.p[og] : real_t(0), cz = own_in ? gz.p[og] : real_t(0);
+  real_t cg = active ? gmag.p[og] : real_t(0);
+  real_t vp = hp >= 0 ? F.p[hp] : real_t(0), vx = hx >= 0 ? gx.p[hx] : real_t(0),
+         vz = hz >= 0 ? gz.p[hz] : real_t(0);
+  for (int k = kb; k <= ke; k++) {
+    const int d = k - kb;
+    const bool more = k < ke;
+    // plane k + 3 of the columns, plane k + 1 of the rest
+    p[5] = own_in && more ? F.p[op + (d + 3) * psz] : real_t(0);
+    y[5] = active && more ? gy.p[og + (d + 3) * gsz] : real_t(0);
+    const real_t ncx = own_in && more ? gx.p[og + (d + 1) * gsz] : real_t(0);
+    const real_t ncz = own_in && more ? gz.p[og + (d + 1) * gsz] : real_t(0);
+    const real_t ncg = active && more ? gmag.p[og + (d + 1) * gsz] : real_t(0);
+    const real_t nvp = hp >= 0 && more ? F.p[hp + (d + 1) * psz] : real_t(0);
+    const real_t nvx = hx >= 0 && more ? gx.p[hx + (d + 1) * gsz] : real_t(0);
+    const real_t nvz = hz >= 0 && more ? gz.p[hz + (d + 1) * gsz] : real_t(0);
+    __syncthreads();  // the previous plane's LDS reads are done
+    P[slp] = p[2];
+    X[slx] = cx;
+    Z[slz] = cz;
+    if (tid < NP) P[sp] = vp;
+    if (tid < NX) X[sx] = vx;
+    if (tid < NZ) Z[szs] = vz;
+    __syncthreads();
+    if (active) {
+      const real_t fxm2 = P[slp - 2], fxm1 = P[slp - 1], fxp1 = P[slp + 1], fxp2 = P[slp + 2];
+      const real_t fym2 = P[slp - 2 * PX], fym1 = P[slp - PX], fyp1 = P[slp + PX], fyp2 = P[slp + 2 * PX];
+      const real_t xym2 = X[slx - 2 * MX], xym1 = X[slx - MX], xyp1 = X[slx + MX], xyp2 = X[slx + 2 * MX];
+      const real_t zxm2 = Z[slz - 2], zxm1 = Z[slz - 1], zxp1 = Z[slz + 1], zxp2 = Z[slz + 2];
+      const real_t x = cx, yc = y[2], z = cz, gm = cg;
+      const real_t xx = (-fxp2 + real_t(16) * fxp1 - real_t(30) * p[2] - fxm2 + real_t(16) * fxm1) * f.xx;
+      const real_t yy = (-fyp2 + real_t(16) * fyp1 - real_t(30) * p[2] - fym2 + real_t(16) * fym1) * f.yy;
+      const real_t zz = (-p[4] + real_t(16) * p[3] - real_t(30) * p[2] - p[0] + real_t(16) * p[1]) * f.zz;
+      const real_t xy = (-xyp2 + real_t(8) * xyp1 + xym2 - real_t(8) * xym1) * f.y;
+      const real_t yz = (-y[4] + real_t(8) * y[3] + y[0] - real_t(8) * y[1]) * f.z;
+      const real_t zx = (-zxp2 + real_t(8) * zxp1 + zxm2 - real_t(8) * zxm1) * f.x;
+      const real_t den = GAUSS ? gm * gm * gm * gm : gm * gm * gm;
+      const real_t num = GAUSS ? gauss_num(x, yc, z, xx, yy, zz, xy, zx, yz)
+                               : stencil::curv_num(xx, xy, zx, yy, yz, zz, x, yc, z);
+      const real_t quot = num / den;
+      kappa(i, j, k) = den < kZeroTol ? real_t(0) : quot;
+    }
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+      p[s] = p[s + 1];
+      y[s] = y[s + 1];
+    }
+    cx = ncx;
+    cz = ncz;
+    cg = ncg;
+    vp = nvp;
+    vx = nvx;
+    vz = nvz;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -494,9 +567,10 @@ static void curvature_order4(const char* who, real_t* kappa, const Box& kg, cons
     return;
   const real_t c = real_t(0.0833333333333333333333);
   Factors4 f{c / dx / dx, c / dy / dy, c / dz / dz, c / dx, c / dy, c / dz};
-  const Tiles t = tiles(fb, kMarchKC);
-  hipLaunchKernelGGL((curvature_order4_kernel<kMarchKC, GAUSS>), tile_grid(t), block2d(), 0, stream(), t, kappa,
-                     lay(kg), phi, lay(pg), px, py, pz, gmag, lay(gg), f);
+  const lsmhip::march::Tiles tl = lsmhip::march::tiles_for(fb);
+  hipLaunchKernelGGL((curvature_order4_march_kernel<GAUSS>), lsmhip::march::grid_for(tl), lsmhip::march::block(), 0,
+                     stream(), fb, view(phi, pg), view(px, gg), view(py, gg), view(pz, gg), view(gmag, gg),
+                     view(kappa, kg), tl, f);
   finish(who);
 }
 

@@ -56,6 +56,20 @@ def test_neighbour_kernels_bitwise(hip, sym, seed):
         H.assert_same(want, got, sym)
 
 
+@pytest.mark.parametrize("sym", sorted(ORDER4) + ["lsm3dcomputegaussiancurvatureorder2_"])
+def test_plane_march_over_many_tiles_and_chunks(hip, sym):
+    """The curvatures on the plane march (64 x 8 tiles, chunks of 16 planes),
+    on a fill box of 141 x 21 x 38 cells: three tiles along i (the last one
+    13 wide), three along j (the last 5), three chunks along k (the last 6),
+    phi and the gradient on ghost boxes three cells wider."""
+    c = H.grid_case(8, gb=(-3, 143, -3, 23, -3, 40), fb=(0, 140, 0, 20, 0, 37))
+    prepare_neighbours(c, sym)
+    want = H.run(H.oracle(), sym, c)
+    got = H.run(hip.api, sym, c, device="cuda")
+    hip.check()
+    H.assert_same(want, got, sym)
+
+
 def test_three_halves_power_over_a_wide_range(hip):
     """The mean curvature's (|grad phi|^2)^1.5 over 4M cells whose |grad phi|^2
     spans 1e-11 .. 1e220 (past the overflow of the power at 1.8e205), against the oracle's
