@@ -88,12 +88,18 @@ void face(const char* who, real_t* phi, const Box& gb, const Box& fb, int idx, i
 // on that axis is the edge or next-inner layer; those cells' own values come
 // from the faces of the remaining ghost axes, all applied earlier.  So the final
 // value of a ghost cell is that recursion evaluated from interior cells only,
-// with the face kernel's operations; one thread per ghost cell computes it
-// and no thread reads a ghost cell.
+// with the face kernel's operations, and no ghost cell is read.
+//
+// The launch covers the ghost shell by lines normal to a face: one thread per
+// (i, j) column of the ghost box for the k ghost layers, one per (i, k) with
+// k interior for the j ghost layers, and eight lanes per (j, k) row with j and
+// k interior for the i ghost layers (adjacent lanes write adjacent cells, so a
+// row's few ghost cells go out in one store request).  A line reads its edge
+// and next-inner values once and writes every ghost layer on that side from
+// them.
 struct Shell {
   Box gb, fb;
-  long long nzr, nyr;  // cells in the z slabs (k ghost) and in the y slabs (k interior, j ghost)
-  long long total;
+  long long lz, ly, lx;  // lines normal to the z, y and x faces
   int kind;
 };
 
@@ -103,8 +109,8 @@ __device__ __forceinline__ real_t extrap(real_t pb, real_t pin, int dist, int ki
   return pb + slope * real_t(dist);
 }
 
-// Value of cell (i, j, k) once the faces of axis 0 (v0: j, k interior), of
-// axes 0..1 (v1: k interior) and of all three axes (v2) have been applied.
+// Value of cell (i, j, k) once the faces of axis 0 (v0: j, k interior) and of
+// axes 0..1 (v1: k interior) have been applied.
 __device__ __forceinline__ real_t shell_v0(const View<real_t>& P, const Shell& S, int i, int j, int k) {
   const Box& f = S.fb;
   if (i >= f.ilo && i <= f.ihi) return P(i, j, k);
@@ -122,52 +128,55 @@ __device__ __forceinline__ real_t shell_v1(const View<real_t>& P, const Shell& S
   const real_t pb = shell_v0(P, S, i, e, k);
   return extrap(pb, S.kind == kCopy ? real_t(0) : shell_v0(P, S, i, n, k), up ? j - e : e - j, S.kind);
 }
-__device__ __forceinline__ real_t shell_v2(const View<real_t>& P, const Shell& S, int i, int j, int k) {
-  const Box& f = S.fb;
-  if (k >= f.klo && k <= f.khi) return shell_v1(P, S, i, j, k);
-  const bool up = k > f.khi;
-  if (S.kind == kPeriodic) return shell_v1(P, S, i, j, up ? k - f.nz() : k + f.nz());
-  const int e = up ? f.khi : f.klo, n = up ? f.khi - 1 : f.klo + 1;
-  const real_t pb = shell_v1(P, S, i, j, e);
-  return extrap(pb, S.kind == kCopy ? real_t(0) : shell_v1(P, S, i, j, n), up ? k - e : e - k, S.kind);
-}
 
 __global__ void __launch_bounds__(256) shell_kernel(View<real_t> P, Shell S) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= S.total) return;
   const Box &g = S.gb, &f = S.fb;
-  const int nxg = g.nx(), nyg = g.ny();
-  int i, j, k;
-  if (t < S.nzr) {  // k ghost, i and j over the ghost box
-    const long long plane = (long long)nxg * nyg;
-    const int kk = (int)(t / plane);
-    const int below = f.klo - g.klo;
-    k = kk < below ? g.klo + kk : f.khi + 1 + (kk - below);
-    const long long r = t % plane;
-    j = g.jlo + (int)(r / nxg);
-    i = g.ilo + (int)(r % nxg);
-  } else if (t < S.nzr + S.nyr) {  // k interior, j ghost, i over the ghost box
-    const long long u = t - S.nzr;
-    const int ngy = (f.jlo - g.jlo) + (g.jhi - f.jhi);
-    const long long row = (long long)nxg * ngy;
-    k = f.klo + (int)(u / row);
-    const long long r = u % row;
-    const int jj = (int)(r / nxg);
-    const int below = f.jlo - g.jlo;
-    j = jj < below ? g.jlo + jj : f.jhi + 1 + (jj - below);
-    i = g.ilo + (int)(r % nxg);
-  } else {  // k and j interior, i ghost
-    const long long u = t - S.nzr - S.nyr;
-    const int ngx = (f.ilo - g.ilo) + (g.ihi - f.ihi);
-    const long long row = (long long)ngx * f.ny();
-    k = f.klo + (int)(u / row);
-    const long long r = u % row;
-    j = f.jlo + (int)(r / ngx);
-    const int ii = (int)(r % ngx);
-    const int below = f.ilo - g.ilo;
-    i = ii < below ? g.ilo + ii : f.ihi + 1 + (ii - below);
+  if (t < S.lz) {  // column (i, j): the k ghost layers
+    const int nxg = g.nx(), c = (int)t;
+    const int i = g.ilo + c % nxg, j = g.jlo + c / nxg;
+    for (int up = 0; up < 2; up++) {
+      const int k0 = up ? f.khi + 1 : g.klo, k1 = up ? g.khi : f.klo - 1;
+      if (S.kind == kPeriodic) {
+        for (int k = k0; k <= k1; k++) P(i, j, k) = shell_v1(P, S, i, j, up ? k - f.nz() : k + f.nz());
+      } else if (k0 <= k1) {
+        const int e = up ? f.khi : f.klo, n = up ? f.khi - 1 : f.klo + 1;
+        const real_t pb = shell_v1(P, S, i, j, e);
+        const real_t pin = S.kind == kCopy ? real_t(0) : shell_v1(P, S, i, j, n);
+        for (int k = k0; k <= k1; k++) P(i, j, k) = extrap(pb, pin, up ? k - e : e - k, S.kind);
+      }
+    }
+  } else if (t < S.lz + S.ly) {  // row (i, k), k interior: the j ghost layers
+    const int nxg = g.nx(), c = (int)(t - S.lz);
+    const int i = g.ilo + c % nxg, k = f.klo + c / nxg;
+    for (int up = 0; up < 2; up++) {
+      const int j0 = up ? f.jhi + 1 : g.jlo, j1 = up ? g.jhi : f.jlo - 1;
+      if (S.kind == kPeriodic) {
+        for (int j = j0; j <= j1; j++) P(i, j, k) = shell_v0(P, S, i, up ? j - f.ny() : j + f.ny(), k);
+      } else if (j0 <= j1) {
+        const int e = up ? f.jhi : f.jlo, n = up ? f.jhi - 1 : f.jlo + 1;
+        const real_t pb = shell_v0(P, S, i, e, k);
+        const real_t pin = S.kind == kCopy ? real_t(0) : shell_v0(P, S, i, n, k);
+        for (int j = j0; j <= j1; j++) P(i, j, k) = extrap(pb, pin, up ? j - e : e - j, S.kind);
+      }
+    }
+  } else {  // row (j, k), j and k interior: the i ghost layers, eight lanes per row
+    const long long u = t - S.lz - S.ly;
+    if (u >= 8 * S.lx) return;
+    const int r = (int)(u >> 3), q = (int)(u & 7), nyi = f.ny();
+    const int j = f.jlo + r % nyi, k = f.klo + r / nyi;
+    const int below = f.ilo - g.ilo, depth = below + (g.ihi - f.ihi);
+    for (int m = q; m < depth; m += 8) {
+      const bool up = m >= below;
+      const int i = up ? f.ihi + 1 + (m - below) : g.ilo + m;
+      if (S.kind == kPeriodic) {
+        P(i, j, k) = P(up ? i - f.nx() : i + f.nx(), j, k);
+      } else {
+        const int e = up ? f.ihi : f.ilo, n = up ? f.ihi - 1 : f.ilo + 1;
+        P(i, j, k) = extrap(P(e, j, k), S.kind == kCopy ? real_t(0) : P(n, j, k), up ? i - e : e - i, S.kind);
+      }
+    }
   }
-  P(i, j, k) = shell_v2(P, S, i, j, k);
 }
 
 // Checks of face() for all six faces, then one shell launch.
@@ -189,12 +198,16 @@ int shell(const char* who, real_t* phi, const Box& gb, const Box& fb, int kind) 
       return -1;
     }
   }
+  // lines exist only normal to faces with ghost layers
+  const bool gz = fb.klo > gb.klo || fb.khi < gb.khi, gy = fb.jlo > gb.jlo || fb.jhi < gb.jhi;
+  const bool gx = fb.ilo > gb.ilo || fb.ihi < gb.ihi;
   Shell S{gb, fb, 0, 0, 0, kind};
-  S.nzr = (long long)((fb.klo - gb.klo) + (gb.khi - fb.khi)) * gb.ny() * gb.nx();
-  S.nyr = (long long)fb.nz() * ((fb.jlo - gb.jlo) + (gb.jhi - fb.jhi)) * gb.nx();
-  S.total = S.nzr + S.nyr + (long long)fb.nz() * fb.ny() * ((fb.ilo - gb.ilo) + (gb.ihi - fb.ihi));
-  if (S.total == 0) return 0;
-  hipLaunchKernelGGL(shell_kernel, dim3((unsigned)((S.total + 255) / 256)), dim3(256), 0, stream(), view(phi, gb), S);
+  S.lz = gz ? (long long)gb.nx() * gb.ny() : 0;
+  S.ly = gy ? (long long)gb.nx() * fb.nz() : 0;
+  S.lx = gx ? (long long)fb.ny() * fb.nz() : 0;
+  const long long threads = S.lz + S.ly + 8 * S.lx;
+  if (threads == 0) return 0;
+  hipLaunchKernelGGL(shell_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream(), view(phi, gb), S);
   return finish(who) ? 0 : -1;
 }
 

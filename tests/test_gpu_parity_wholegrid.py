@@ -182,9 +182,15 @@ FACE_SYM = {0: "lsm3dlinearextrapolation", 1: "lsm3dsignedlinearextrapolation", 
             3: "lsmhip_periodic_face"}
 
 
+# ghost box, fill box: asymmetric widths; then ten layers below x (more than
+# the eight lanes a row of x ghost cells gets), none on y, one on each z side
+FILL_BOXES = [((-3, 12, -2, 9, -3, 7), (0, 9, 0, 6, 0, 4)), ((-10, 12, 0, 6, -1, 5), (0, 10, 0, 6, 0, 4))]
+
+
+@pytest.mark.parametrize("boxes", FILL_BOXES)
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_fill_all_faces_in_one_launch(kind, prec):
+def test_fill_all_faces_in_one_launch(kind, prec, boxes):
     """lsmhip_fill_ghosts(idx 9) fills the whole ghost shell in one launch; it
     equals the six faces applied in order x_lo .. z_hi bit for bit (FP64:
     against the oracle; FP32: against the library's own face kernels), from
@@ -193,7 +199,7 @@ def test_fill_all_faces_in_one_launch(kind, prec):
     import torch
     import lsmlib_amd
     lib = lsmlib_amd.load(prec)
-    gb, fb = (-3, 12, -2, 9, -3, 7), (0, 9, 0, 6, 0, 4)
+    gb, fb = boxes
     rng = np.random.default_rng(kind)
     phi = rng.standard_normal(H.shape_of(gb))
     box = [ctypes.POINTER(ctypes.c_int)] * 6
