@@ -316,8 +316,10 @@ LSMHIP_EXPORT void lsm3dlaplacianorder2_(real_t* lap, LSMHIP_BOXP(lg), const rea
       !check_size(lg, who) || !check_size(pg, who))
     return;
   const real_t ax = real_t(1.0) / *dx / *dx, ay = real_t(1.0) / *dy / *dy, az = real_t(1.0) / *dz / *dz;
-  const march::Tiles tl = march::tiles_for(fb);
-  hipLaunchKernelGGL(march::star_march_kernel<Laplacian2Star>, march::grid_for(tl), march::block(), 0, stream(), fb,
+  // 128-wide tiles: 315 -> 253 us FP32 and unchanged FP64 at 512^3 (the central
+  // gradient, with three outputs, gained nothing from them and keeps 64-wide tiles)
+  const march::Tiles tl = march::tiles_for_x2(fb);
+  hipLaunchKernelGGL(march::star_march_x2_kernel<Laplacian2Star>, march::grid_for(tl), march::block(), 0, stream(), fb,
                      view(phi, pg), tl, Laplacian2Star{view(lap, lg), ax, ay, az});
   finish(who);
 }

@@ -113,6 +113,80 @@ __global__ void __launch_bounds__(kMX* kMY) star_march_kernel(Box fb, View<const
   }
 }
 
+// star_march_kernel over tiles twice as wide: each thread owns the columns
+// i0 + x and i0 + kMX + x of a 2 kMX x kMY tile.  Per cache-line request a
+// plane row then carries 512 rather than 256 bytes of FP32 (1024 / 512 of
+// FP64), and the two one-cell halo columns serve twice the cells.
+inline Tiles tiles_for_x2(const Box& fb) {
+  return Tiles{(fb.nx() + 2 * kMX - 1) / (2 * kMX), (fb.ny() + kMY - 1) / kMY, (fb.nz() + kChunk - 1) / kChunk, kChunk};
+}
+template <class Op>
+__global__ void __launch_bounds__(kMX* kMY) star_march_x2_kernel(Box fb, View<const real_t> phi, Tiles tl, Op op) {
+  constexpr int TW = 2 * kMX, PX = TW + 2, PLANE = PX * (kMY + 2), HALO = PLANE - TW * kMY;
+  __shared__ real_t plane[PLANE];
+  static_assert(HALO <= kMX * kMY, "one halo element per thread at most");
+  const int tid = threadIdx.y * kMX + threadIdx.x;
+  const int nt = tl.nti * tl.ntj * tl.ntk;
+  const int per = (nt + 7) / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= nt) return;
+  const int ti = t % tl.nti, tj = (t / tl.nti) % tl.ntj, tk = t / (tl.nti * tl.ntj);
+  const int i0 = fb.ilo + ti * TW, j0 = fb.jlo + tj * kMY;
+  const int kb = fb.klo + tk * tl.kc;
+  const int ke = min(kb + tl.kc - 1, fb.khi);
+  const int ia = i0 + (int)threadIdx.x, ib = ia + kMX, j = j0 + (int)threadIdx.y;
+  const bool act_a = ia <= fb.ihi && j <= fb.jhi, act_b = ib <= fb.ihi && j <= fb.jhi;
+  const bool in_a = ia <= fb.ihi + 1 && j <= fb.jhi + 1, in_b = ib <= fb.ihi + 1 && j <= fb.jhi + 1;
+  const int own_a = in_a ? phi.offset(ia, j, kb) : 0, own_b = in_b ? phi.offset(ib, j, kb) : 0;
+  // halo: the row above and the row below (PX cells each), then the cells left
+  // and right of each interior row
+  int hoff = -1, hslot = 0;
+  if (tid < HALO) {
+    int px, py;
+    if (tid < 2 * PX) {
+      py = tid < PX ? 0 : kMY + 1;
+      px = tid % PX;
+    } else {
+      const int h = tid - 2 * PX;
+      py = 1 + h / 2;
+      px = h % 2 ? TW + 1 : 0;
+    }
+    const int gi = i0 - 1 + px, gj = j0 - 1 + py;
+    hslot = py * PX + px;
+    if (gi <= fb.ihi + 1 && gj <= fb.jhi + 1) hoff = phi.offset(gi, gj, kb);
+  }
+  const int sz = phi.sz;
+  const int sa = (threadIdx.y + 1) * PX + threadIdx.x + 1, sb = sa + kMX;
+  // za[s] = phi(ia, j, k - 1 + s), zb likewise, h[s] = the halo element at plane k + s
+  real_t za[4], zb[4], h[2];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    za[s] = in_a && kb + s - 1 <= fb.khi + 1 ? phi.p[own_a + (s - 1) * sz] : real_t(0);
+    zb[s] = in_b && kb + s - 1 <= fb.khi + 1 ? phi.p[own_b + (s - 1) * sz] : real_t(0);
+  }
+  h[0] = hoff >= 0 ? phi.p[hoff] : real_t(0);
+  for (int k = kb; k <= ke; k++) {
+    const int dk = (k - kb) * sz;
+    // the next plane, in flight while this plane is worked on
+    za[3] = in_a && k + 2 <= fb.khi + 1 ? phi.p[own_a + dk + 2 * sz] : real_t(0);
+    zb[3] = in_b && k + 2 <= fb.khi + 1 ? phi.p[own_b + dk + 2 * sz] : real_t(0);
+    h[1] = hoff >= 0 && k + 1 <= ke ? phi.p[hoff + dk + sz] : real_t(0);
+    __syncthreads();  // the previous plane's LDS reads are done
+    plane[sa] = za[1];
+    plane[sb] = zb[1];
+    if (tid < HALO) plane[hslot] = h[0];
+    __syncthreads();
+    if (act_a) op(ia, j, k, za[1], plane[sa - 1], plane[sa + 1], plane[sa - PX], plane[sa + PX], za[0], za[2]);
+    if (act_b) op(ib, j, k, zb[1], plane[sb - 1], plane[sb + 1], plane[sb - PX], plane[sb + PX], zb[0], zb[2]);
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      za[s] = za[s + 1];
+      zb[s] = zb[s + 1];
+    }
+    h[0] = h[1];
+  }
+}
+
 // The 19 values of the compact second-order cross around (i, j, k): the cell,
 // its six face neighbours and the twelve edge neighbours, the edges of each
 // plane pair ordered (+,+), (+,-), (-,+), (-,-).

@@ -140,10 +140,11 @@ PLANE_MARCH = ["lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlapla
 
 @pytest.mark.parametrize("sym", PLANE_MARCH)
 def test_plane_march_over_many_tiles_and_chunks(hip, sym):
-    """The kernels that march 64 x 8 tiles through chunks of 16 planes, on a
-    fill box of 141 x 21 x 38 cells: three tiles along i (the last one 13
-    wide), three along j (the last 5), three chunks along k (the last 6);
-    phi's ghost box is three cells wider than the fill box on every side."""
+    """The kernels that march 64 x 8 tiles (128 x 8 for the Laplacian)
+    through chunks of 16 planes, on a fill box of 141 x 21 x 38 cells: three
+    tiles along i (two of 128), the last one 13 wide, three along j (the last
+    5), three chunks along k (the last 6); phi's ghost box is three cells
+    wider than the fill box on every side."""
     c = H.grid_case(6, gb=(-3, 143, -3, 23, -3, 40), fb=(0, 140, 0, 20, 0, 37))
     _prepare(c, sym)
     _cmp(hip, sym, c)
