@@ -56,13 +56,17 @@ def test_neighbour_kernels_bitwise(hip, sym, seed):
         H.assert_same(want, got, sym)
 
 
+@pytest.mark.parametrize("boxes", [((-3, 143, -3, 23, -3, 40), (0, 140, 0, 20, 0, 37)),
+                                   ((-3, 131, -3, 11, -3, 19), (0, 128, 0, 8, 0, 16))])
 @pytest.mark.parametrize("sym", sorted(ORDER4) + ["lsm3dcomputegaussiancurvatureorder2_"])
-def test_plane_march_over_many_tiles_and_chunks(hip, sym):
-    """The curvatures on the plane march (64 x 8 tiles, chunks of 16 planes),
-    on a fill box of 141 x 21 x 38 cells: three tiles along i (the last one
-    13 wide), three along j (the last 5), three chunks along k (the last 6),
-    phi and the gradient on ghost boxes three cells wider."""
-    c = H.grid_case(8, gb=(-3, 143, -3, 23, -3, 40), fb=(0, 140, 0, 20, 0, 37))
+def test_plane_march_over_many_tiles_and_chunks(hip, sym, boxes):
+    """The curvatures on the plane march (64 x 8 tiles, chunks of 16 planes).
+    On a fill box of 141 x 21 x 38 cells: three tiles along i (the last one
+    13 wide), three along j (the last 5), three chunks along k (the last 6).
+    On 129 x 9 x 17 cells every last tile and chunk is a single cell.  phi and
+    the gradient on ghost boxes three cells wider."""
+    gb, fb = boxes
+    c = H.grid_case(8, gb=gb, fb=fb)
     prepare_neighbours(c, sym)
     want = H.run(H.oracle(), sym, c)
     got = H.run(hip.api, sym, c, device="cuda")

@@ -138,14 +138,23 @@ PLANE_MARCH = ["lsm3dcentralgradorder2_", "lsm3dcentralgradorder4_", "lsm3dlapla
                "lsm3dupwindhjeno2_", "lsm3dupwindhjeno3_", "lsm3dupwindhjweno5_"]
 
 
+# (ghost box, fill box): 141 x 21 x 38 cells; then 129 x 9 x 17, whose last
+# tile along i (for 64- and 128-wide tiles), along j and along k is one cell
+MANY_TILE_BOXES = [((-3, 143, -3, 23, -3, 40), (0, 140, 0, 20, 0, 37)),
+                   ((-3, 131, -3, 11, -3, 19), (0, 128, 0, 8, 0, 16))]
+
+
+@pytest.mark.parametrize("boxes", MANY_TILE_BOXES)
 @pytest.mark.parametrize("sym", PLANE_MARCH)
-def test_plane_march_over_many_tiles_and_chunks(hip, sym):
+def test_plane_march_over_many_tiles_and_chunks(hip, sym, boxes):
     """The kernels that march 64 x 8 tiles (128 x 8 for the Laplacian)
-    through chunks of 16 planes, on a fill box of 141 x 21 x 38 cells: three
-    tiles along i (two of 128), the last one 13 wide, three along j (the last
-    5), three chunks along k (the last 6); phi's ghost box is three cells
-    wider than the fill box on every side."""
-    c = H.grid_case(6, gb=(-3, 143, -3, 23, -3, 40), fb=(0, 140, 0, 20, 0, 37))
+    through chunks of 16 planes.  On the 141 x 21 x 38 fill box: three tiles
+    along i (two of 128), the last one 13 wide, three along j (the last 5),
+    three chunks along k (the last 6).  On the 129 x 9 x 17 box every last
+    tile and chunk is a single cell.  phi's ghost box is three cells wider
+    than the fill box on every side."""
+    gb, fb = boxes
+    c = H.grid_case(6, gb=gb, fb=fb)
     _prepare(c, sym)
     _cmp(hip, sym, c)
 
