@@ -302,8 +302,11 @@ template <int RHS, int STAGE, class F>
 __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
   using namespace stencil::fastf;
   __shared__ F plane[PLANE];
-  __shared__ Win ywin[TY][TX];
-  __shared__ Win xwin[TY][TX + 1];  // column TX: the windows one cell past the tile
+  // the plus derivative of the cell before (x) and the row below (y), formed by
+  // the cell that owns the window it comes from; column TX of xplus: the
+  // cells one past the tile
+  __shared__ F yplus[TY][TX];
+  __shared__ F xplus[TY][TX + 1];
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int nt = a.nti * a.ntj * a.ntk;
   const int b = blockIdx.x;
@@ -340,7 +343,10 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
   F dz[5];  // dz[t] = D(k-1+t), D(m) = u(m) - u(m-1) along z
 #pragma unroll
   for (int q = 0; q < 4; q++) dz[q] = zw[q + 2] - zw[q + 1];
-  Win wz = win5(zw[1] - zw[0], dz[0], dz[1], dz[2], dz[3]);  // window D(k-2..k+2) of the first plane
+  // minus derivative of the first plane, from its window D(k-2..k+2)
+  F zm_next, zp_unused;
+  fwd_rev(win5(zw[1] - zw[0], dz[0], dz[1], dz[2], dz[3]), dz[0], dz[1], dz[2], zm_next, zp_unused);
+  zm_next *= iz;
 
   for (int k = kb; k <= ke; k++) {
     const long long zoff = sz * (k - gb.klo);
@@ -357,13 +363,17 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
     plane[ly * PX + lx] = zw[3];
     if (tid < NHALO) plane[hslot] = hv;
     __syncthreads();
-    // z: plus(k) from the window D(k-1..k+3), minus(k) from the carried D(k-2..k+2)
+    // z: the window D(k-1..k+3) gives plus(k) and minus(k+1); minus(k) was
+    // formed from the previous plane's window
     dz[4] = zw[6] - zw[5];
     const Win wz1 = win5(dz[0], dz[1], dz[2], dz[3], dz[4]);
-    const float zp = rev(wz1, dz[1], dz[2], dz[3]) * iz;
-    const float zm = fwd(wz, dz[0], dz[1], dz[2]) * iz;
-    wz = wz1;
-    // x
+    float zpf, zmf;
+    fwd_rev(wz1, dz[1], dz[2], dz[3], zmf, zpf);
+    const float zp = zpf * iz;
+    const float zm = zm_next;
+    zm_next = zmf * iz;
+    // x: this cell's window D(i-2..i+2) gives its minus derivative and the plus
+    // derivative of the cell before it
     F dx_[6], dy_[6];
 #pragma unroll
     for (int q = -2; q <= 3; q++) {
@@ -371,24 +381,35 @@ __global__ void __launch_bounds__(TX* TY) fused_pm_f32_kernel(FusedArgs a) {
       dy_[q + 2] = plane[(ly + q) * PX + lx] - plane[(ly + q - 1) * PX + lx];
     }
     const Win wx = win5(dx_[0], dx_[1], dx_[2], dx_[3], dx_[4]);
-    const float xm = fwd(wx, dx_[1], dx_[2], dx_[3]) * ix;
-    xwin[threadIdx.y][threadIdx.x] = wx;
-    if (threadIdx.y == 0 && threadIdx.x < TY) {  // one wave forms the TY windows past the tile's last column
+    float xmf, xpf;
+    fwd_rev(wx, dx_[1], dx_[2], dx_[3], xmf, xpf);
+    const float xm = xmf * ix;
+    xplus[threadIdx.y][threadIdx.x] = xpf * ix;
+    if (threadIdx.y == 0 && threadIdx.x < TY) {  // one wave forms the TY plus derivatives of the tile's last column
       const int row = (threadIdx.x + H) * PX + TX + H;
       F e[5];
 #pragma unroll
       for (int q = -2; q <= 2; q++) e[q + 2] = plane[row + q] - plane[row + q - 1];
-      xwin[threadIdx.x][TX] = win5(e[0], e[1], e[2], e[3], e[4]);
+      float ef, er;
+      fwd_rev(win5(e[0], e[1], e[2], e[3], e[4]), e[1], e[2], e[3], ef, er);
+      xplus[threadIdx.x][TX] = er * ix;
     }
-    // y
+    // y: likewise along j
     const Win wy = win5(dy_[0], dy_[1], dy_[2], dy_[3], dy_[4]);
-    const float ym = fwd(wy, dy_[1], dy_[2], dy_[3]) * iy;
-    ywin[threadIdx.y][threadIdx.x] = wy;
+    float ymf, ypf;
+    fwd_rev(wy, dy_[1], dy_[2], dy_[3], ymf, ypf);
+    const float ym = ymf * iy;
+    yplus[threadIdx.y][threadIdx.x] = ypf * iy;
     __syncthreads();
-    const Win wyn = threadIdx.y == TY - 1 ? win5(dy_[1], dy_[2], dy_[3], dy_[4], dy_[5])
-                                          : ywin[threadIdx.y + 1][threadIdx.x];
-    const float yp = rev(wyn, dy_[2], dy_[3], dy_[4]) * iy;
-    const float xp = rev(xwin[threadIdx.y][threadIdx.x + 1], dx_[2], dx_[3], dx_[4]) * ix;
+    float yp;
+    if (threadIdx.y == TY - 1) {  // the last row forms the plus derivative from the row past the tile
+      float nf, nr;
+      fwd_rev(win5(dy_[1], dy_[2], dy_[3], dy_[4], dy_[5]), dy_[2], dy_[3], dy_[4], nf, nr);
+      yp = nr * iy;
+    } else {
+      yp = yplus[threadIdx.y + 1][threadIdx.x];
+    }
+    const float xp = xplus[threadIdx.y][threadIdx.x + 1];
     if (active) {
       F r;
       if (RHS == kReinit) {

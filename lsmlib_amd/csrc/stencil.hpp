@@ -332,6 +332,23 @@ __device__ __forceinline__ float rev(const Win& w, float v2, float v3, float v4)
 __device__ __forceinline__ float weno5(float v1, float v2, float v3, float v4, float v5) {
   return fwd(win5(v1, v2, v3, v4, v5), v2, v3, v4);
 }
+// fwd and rev of one window together (the minus derivative at a cell and the
+// plus derivative at the cell before it): the two denominators share 0.6 q13,
+// the numerators the products q23 d12 and q12 d23, the linear parts (5/6) v3.
+// The same weights and candidates as fwd and rev, rounded in another order.
+__device__ __forceinline__ void fwd_rev(const Win& w, float v2, float v3, float v4, float& f, float& r) {
+  const float m13 = 0.6f * w.q13;
+  const float den_f = fmaf(0.1f, w.q23, fmaf(0.3f, w.q12, m13));
+  const float den_r = fmaf(0.1f, w.q12, fmaf(0.3f, w.q23, m13));
+  const float pa = w.q23 * w.d12, pb = w.q12 * w.d23;
+  const float num_f = fmaf(0.1f / 3.f, pa, 0.05f * pb);
+  const float num_r = -fmaf(0.1f / 3.f, pb, 0.05f * pa);
+  const float t3 = (5.f / 6.f) * v3;
+  const float p2_f = fmaf(-1.f / 6.f, v2, fmaf(1.f / 3.f, v4, t3));
+  const float p2_r = fmaf(-1.f / 6.f, v4, fmaf(1.f / 3.f, v2, t3));
+  f = fmaf(num_f, rcp(den_f), p2_f);
+  r = fmaf(num_r, rcp(den_r), p2_r);
+}
 
 // Plus and minus WENO5 derivatives from seven samples f[s+3] along an axis:
 // minus uses D(-2..2), plus uses D(3..-1) = the window D(-1..3) reversed.
