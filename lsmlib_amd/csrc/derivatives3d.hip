@@ -177,6 +177,113 @@ __global__ void __launch_bounds__(256) grad_mag_kernel(Box fb, Axes3 g, View<rea
 
 Axes3 axes(real_t* x, real_t* y, real_t* z, const Box& b) { return Axes3{{view(x, b), view(y, b), view(z, b)}}; }
 
+// FP32 HJ-WENO5 plus and minus derivatives on the plane march, each WENO5
+// window formed once.  The minus derivative at a cell and the plus derivative
+// at the cell before it along an axis come from one window (the cell's own),
+// so each cell forms its window once per axis, takes both derivatives from it
+// (stencil::fastf::fwd_rev), and passes the plus derivative to the cell before
+// it: through LDS along x and y, carried in a register along z.  One wave forms
+// the plus derivatives of the cells one column past the tile; the last row
+// forms those of the row past it.  F is float: a template parameter so that the
+// FP64 build, which never launches this kernel, does not compile it.
+template <class F>
+__global__ void __launch_bounds__(march::kMX* march::kMY)
+    weno5_pm_own_window_kernel(Box fb, View<const F> phi, march::Tiles tl, Axes3 plus, Axes3 minus, F ix, F iy, F iz) {
+  using namespace stencil::fastf;
+  constexpr int R = 3, MX = march::kMX, MY = march::kMY, PX = MX + 2 * R, PLANE = PX * (MY + 2 * R);
+  constexpr int HALO = PLANE - MX * MY;
+  static_assert(HALO <= MX * MY, "one halo element per thread at most");
+  __shared__ F plane[PLANE];
+  __shared__ F yplus[MY][MX];
+  __shared__ F xplus[MY][MX + 1];  // column MX: the cells one past the tile
+  const int tid = threadIdx.y * MX + threadIdx.x;
+  const int nt = tl.nti * tl.ntj * tl.ntk;
+  const int per = (nt + 7) / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= nt) return;
+  const int ti = t % tl.nti, tj = (t / tl.nti) % tl.ntj, tk = t / (tl.nti * tl.ntj);
+  const int i0 = fb.ilo + ti * MX, j0 = fb.jlo + tj * MY;
+  const int kb = fb.klo + tk * tl.kc;
+  const int ke = min(kb + tl.kc - 1, fb.khi);
+  const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+  const bool active = i <= fb.ihi && j <= fb.jhi;
+  // columns within R cells of fb lie inside phi's box; further out they are never read
+  const bool own_in = i <= fb.ihi + R && j <= fb.jhi + R;
+  const int own = own_in ? phi.offset(i, j, kb) : 0;
+  int hoff = -1, hslot = 0;
+  if (tid < HALO) {
+    int px, py;
+    march::halo_pos<R>(tid, px, py);
+    const int gi = i0 - R + px, gj = j0 - R + py;
+    hslot = py * PX + px;
+    if (gi <= fb.ihi + R && gj <= fb.jhi + R) hoff = phi.offset(gi, gj, kb);
+  }
+  const int sz = phi.sz;
+  const int slot = (threadIdx.y + R) * PX + threadIdx.x + R;
+  // z[s] = phi(i, j, k - 3 + s); z[7] = plane k + 4 in flight
+  F z[8];
+#pragma unroll
+  for (int s = 0; s < 7; s++) z[s] = own_in ? phi.p[own + (s - R) * sz] : F(0);
+  F h0 = hoff >= 0 ? phi.p[hoff] : F(0);
+  // minus derivative along z at the first plane, from its window D(k-2 .. k+2)
+  F zm, unused;
+  fwd_rev(win5(z[1] - z[0], z[2] - z[1], z[3] - z[2], z[4] - z[3], z[5] - z[4]), z[2] - z[1], z[3] - z[2],
+          z[4] - z[3], zm, unused);
+  for (int k = kb; k <= ke; k++) {
+    const int dk = (k - kb) * sz;
+    z[7] = own_in && k < ke ? phi.p[own + dk + (R + 1) * sz] : F(0);
+    const F h1 = hoff >= 0 && k < ke ? phi.p[hoff + dk + sz] : F(0);
+    __syncthreads();  // the previous plane's LDS reads are done
+    plane[slot] = z[R];
+    if (tid < HALO) plane[hslot] = h0;
+    __syncthreads();
+    F dx[6], dy[6];  // D(i-2 .. i+3), D(j-2 .. j+3)
+#pragma unroll
+    for (int q = -2; q <= 3; q++) {
+      dx[q + 2] = plane[slot + q] - plane[slot + q - 1];
+      dy[q + 2] = plane[slot + q * PX] - plane[slot + (q - 1) * PX];
+    }
+    F xm, xpp, ym, ypp, zmn, zp;
+    fwd_rev(win5(dx[0], dx[1], dx[2], dx[3], dx[4]), dx[1], dx[2], dx[3], xm, xpp);
+    xplus[threadIdx.y][threadIdx.x] = xpp * ix;
+    if (threadIdx.y == 0 && threadIdx.x < MY) {  // the column one past the tile, one row per lane
+      const int row = ((int)threadIdx.x + R) * PX + MX + R;
+      F e[5], ef, er;
+#pragma unroll
+      for (int q = -2; q <= 2; q++) e[q + 2] = plane[row + q] - plane[row + q - 1];
+      fwd_rev(win5(e[0], e[1], e[2], e[3], e[4]), e[1], e[2], e[3], ef, er);
+      xplus[threadIdx.x][MX] = er * ix;
+    }
+    fwd_rev(win5(dy[0], dy[1], dy[2], dy[3], dy[4]), dy[1], dy[2], dy[3], ym, ypp);
+    yplus[threadIdx.y][threadIdx.x] = ypp * iy;
+    // z: the window D(k-1 .. k+3) gives the plus derivative here and the minus one at k + 1
+    fwd_rev(win5(z[2] - z[1], z[3] - z[2], z[4] - z[3], z[5] - z[4], z[6] - z[5]), z[3] - z[2], z[4] - z[3],
+            z[5] - z[4], zmn, zp);
+    __syncthreads();
+    F yp;
+    if (threadIdx.y == MY - 1) {  // the row past the tile
+      F nf, nr;
+      fwd_rev(win5(dy[1], dy[2], dy[3], dy[4], dy[5]), dy[2], dy[3], dy[4], nf, nr);
+      yp = nr * iy;
+    } else {
+      yp = yplus[threadIdx.y + 1][threadIdx.x];
+    }
+    const F xp = xplus[threadIdx.y][threadIdx.x + 1];
+    if (active) {
+      plus.o[0](i, j, k) = xp;
+      plus.o[1](i, j, k) = yp;
+      plus.o[2](i, j, k) = zp * iz;
+      minus.o[0](i, j, k) = xm * ix;
+      minus.o[1](i, j, k) = ym * iy;
+      minus.o[2](i, j, k) = zm * iz;
+    }
+    zm = zmn;
+#pragma unroll
+    for (int s = 0; s < 7; s++) z[s] = z[s + 1];
+    h0 = h1;
+  }
+}
+
 template <int ORDER>
 void hj_plus_minus(const char* who, real_t* pxp, real_t* pyp, real_t* pzp, const Box& gp, real_t* pxm, real_t* pym,
                    real_t* pzm, const Box& gm, const real_t* phi, const Box& pg, const Box& fb, real_t dx, real_t dy,
@@ -188,10 +295,16 @@ void hj_plus_minus(const char* who, real_t* pxp, real_t* pyp, real_t* pzp, const
       !check_size(gm, who))
     return;
   const march::Tiles tl = march::tiles_for(fb);
-  hipLaunchKernelGGL((march::line_march_kernel<w, PlusMinusLine<ORDER>>), march::grid_for(tl), march::block(), 0,
-                     stream(), fb, view(phi, pg), tl,
-                     PlusMinusLine<ORDER>{axes(pxp, pyp, pzp, gp), axes(pxm, pym, pzm, gm),
-                                          {real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz}});
+  if constexpr (ORDER == 5 && sizeof(real_t) == 4) {
+    hipLaunchKernelGGL(weno5_pm_own_window_kernel<real_t>, march::grid_for(tl), march::block(), 0, stream(), fb,
+                       view(phi, pg), tl, axes(pxp, pyp, pzp, gp), axes(pxm, pym, pzm, gm), real_t(1.0) / dx,
+                       real_t(1.0) / dy, real_t(1.0) / dz);
+  } else {
+    hipLaunchKernelGGL((march::line_march_kernel<w, PlusMinusLine<ORDER>>), march::grid_for(tl), march::block(), 0,
+                       stream(), fb, view(phi, pg), tl,
+                       PlusMinusLine<ORDER>{axes(pxp, pyp, pzp, gp), axes(pxm, pym, pzm, gm),
+                                            {real_t(1.0) / dx, real_t(1.0) / dy, real_t(1.0) / dz}});
+  }
   finish(who);
 }
 

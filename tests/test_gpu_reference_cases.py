@@ -140,6 +140,26 @@ def test_f32_whole_grid_within_tolerance(hip32, sym):
     _cmp32(ref, got, sym)
 
 
+@pytest.mark.parametrize("boxes", [((-3, 143, -3, 23, -3, 40), (0, 140, 0, 20, 0, 37)),
+                                   ((-3, 131, -3, 11, -3, 19), (0, 128, 0, 8, 0, 16))])
+def test_f32_weno5_over_many_tiles(hip32, boxes):
+    """FP32 HJ-WENO5 forms each window once and passes the plus derivative to
+    the neighbouring cell (through LDS within a 64 x 8 tile, from the cells one
+    past it at its edges, along k from plane to plane): within the FP32
+    tolerance of the FP64 oracle on fill boxes of three tiles per axis and of
+    a single-cell last tile and chunk on every axis."""
+    sym = "lsm3dhjweno5_"
+    gb, fb = boxes
+    c = H.grid_case(4, gb=gb, fb=fb)  # outputs on ghost boxes of their own
+    _smooth_case(sym, c)
+    _prepare(c, sym)
+    ref = H.run(H.oracle(), sym, c)
+    got = H.run(hip32.api, sym, c, device="cuda", real32=True)
+    hip32.check()
+    worst = _cmp32(ref, got, sym)
+    print("worst relative error %.2e (bound %.0e)" % (worst, F32_RTOL))
+
+
 @pytest.mark.parametrize("sym", LOCAL)
 def test_f32_local_within_tolerance(hip32, sym):
     c = _full_band_case()
