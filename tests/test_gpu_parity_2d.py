@@ -132,6 +132,24 @@ def test_f32_2d_within_tolerance(hip32, sym):
     _cmp32(ref, got, sym)
 
 
+@pytest.mark.parametrize("boxes", [((-3, 143, -3, 23), (0, 140, 0, 20)), ((-3, 131, -3, 11), (0, 128, 0, 8))])
+def test_f32_2d_weno5_over_many_tiles(hip32, boxes):
+    """FP32 2D HJ-WENO5 within the FP32 tolerance of the FP64 oracle on fill
+    boxes of three 64 x 4 tiles along i and six along j, and of a single-cell
+    last tile on both axes, the outputs on ghost boxes of their own."""
+    sym = "lsm2dhjweno5_"
+    gb, fb = boxes
+    c = H.grid_case2d(2, gb=gb, fb=fb)  # outputs on ghost boxes of their own
+    bx = H.box_of_array(sym)
+    for kind, name in capi.SPECS[sym]:
+        if kind == "R":
+            c.arrays[name] = _smooth2d(name, c.box(bx[name]), c.scalars["dx"])
+    ref = H.run(H.oracle(), sym, c)
+    got = H.run(hip32.api, sym, c, device="cuda", real32=True)
+    hip32.check()
+    _cmp32(ref, got, sym)
+
+
 @pytest.mark.parametrize("sym,box,bad", [
     ("lsm2dcomputemeancurvatureorder4_", "g", (-3, 22, -1, 18)),   # phi_x(j-2) outside
     ("lsm2dhjweno5_", "phi", (-2, 22, -3, 18)),                    # phi(i-3) outside
